@@ -1,0 +1,291 @@
+// Device context, plan tables, work buffers and the launch sequence.
+#include "fip_context.hpp"
+
+#include "fip.h"
+#include "fip_plan.hpp"
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <limits>
+
+namespace fip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+int hip_error(hipError_t e, const char* what)
+{
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? FIP_ERR_ALLOC : FIP_ERR_HIP;
+    return set_error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+hipError_t DevBuf::ensure(size_t bytes)
+{
+    if (bytes <= cap && p) return hipSuccess;
+    release();
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        return e;
+    }
+    cap = bytes;
+    return hipSuccess;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+hipError_t PinnedBuf::ensure(size_t bytes)
+{
+    if (bytes <= cap && p) return hipSuccess;
+    release();
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        return e;
+    }
+    cap = bytes;
+    return hipSuccess;
+}
+
+void PinnedBuf::release()
+{
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+// ---------------------------------------------------------------- work
+
+size_t Work::bytes(int S, size_t stride, const PlanTables& p) const
+{
+    const size_t s = (size_t)S;
+    const size_t nvis = s * stride;
+    return nvis * sizeof(VisRec) * 2 + nvis * sizeof(int2) + s * p.G * 4 + s * (p.G + 1) * 4 +
+           s * sizeof(SnapMeta) + s * sizeof(SnapGeom) + s * (size_t)p.I * (p.M + 1) * 8 +
+           2 * s * (size_t)p.I * p.I * 4 + 256;
+}
+
+int Work::ensure(int S, size_t stride, const PlanTables& p)
+{
+    const size_t s = (size_t)S;
+    const size_t nvis = s * stride;
+    FIP_HIP(rec.ensure(nvis * sizeof(VisRec)));
+    FIP_HIP(sorted.ensure(nvis * sizeof(VisRec)));
+    FIP_HIP(bkslot.ensure(nvis * sizeof(int2)));
+    FIP_HIP(hist.ensure(s * p.G * sizeof(int)));
+    FIP_HIP(bstart.ensure(s * (p.G + 1) * sizeof(int)));
+    FIP_HIP(meta.ensure(s * sizeof(SnapMeta)));
+    FIP_HIP(geom.ensure(s * sizeof(SnapGeom)));
+    FIP_HIP(kt.ensure(s * (size_t)p.I * (p.M + 1) * sizeof(float2)));
+    FIP_HIP(pre.ensure(s * (size_t)p.I * p.I * sizeof(float)));
+    FIP_HIP(img.ensure(s * (size_t)p.I * p.I * sizeof(float)));
+    FIP_HIP(work.ensure(256));
+    return FIP_OK;
+}
+
+WorkView Work::view(size_t stride) const
+{
+    WorkView v;
+    v.rec = rec.as<VisRec>();
+    v.bkslot = bkslot.as<int2>();
+    v.sorted = sorted.as<VisRec>();
+    v.hist = hist.as<int>();
+    v.bstart = bstart.as<int>();
+    v.meta = meta.as<SnapMeta>();
+    v.geom = geom.as<SnapGeom>();
+    v.kt = kt.as<float2>();
+    v.pre = pre.as<float>();
+    v.img = img.as<float>();
+    v.work = work.as<unsigned>();
+    v.stride = stride;
+    return v;
+}
+
+void Work::release()
+{
+    for (DevBuf* b : {&rec, &bkslot, &sorted, &hist, &bstart, &meta, &geom, &kt, &pre, &img, &work}) b->release();
+}
+
+// ------------------------------------------------------------- context
+
+int Context::get(Context** out)
+{
+    static std::mutex init_mu;
+    static Context* ctx = nullptr;   // intentionally never destroyed
+    std::lock_guard<std::mutex> lk(init_mu);
+    if (!ctx) {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            return set_error(FIP_ERR_NODEVICE, "no GPU visible to HIP");
+        }
+        auto* c = new Context();
+        FIP_HIP(hipGetDevice(&c->device));
+        FIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto& pair : c->ev_) {
+            FIP_HIP(hipEventCreate(&pair[0]));
+            FIP_HIP(hipEventCreate(&pair[1]));
+        }
+        ctx = c;
+    }
+    *out = ctx;
+    return FIP_OK;
+}
+
+int Context::plan(size_t image, Plan** out)
+{
+    auto it = plans_.find(image);
+    if (it != plans_.end()) {
+        *out = it->second.get();
+        return FIP_OK;
+    }
+    if (!image_size_supported(image)) return set_error(FIP_ERR_SIZE, "unsupported image size");
+    auto P = std::make_unique<Plan>();
+    const int I = (int)image;
+    const int G = (int)grid_size_for(image);
+    const int M = G / 2;
+    P->t.I = I;
+    P->t.G = G;
+    P->t.M = M;
+    std::vector<float2> twg(G), twm(M), wp(M);
+    make_stage_twiddles(G, twg.data());
+    make_stage_twiddles(M, twm.data());
+    for (int k = 0; k < M; ++k) {
+        const double a = 2.0 * M_PI * (double)k / (double)G;
+        wp[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    const std::vector<float> corr = corr_table(image, (size_t)G);
+    FIP_HIP(P->tw_g.ensure(twg.size() * sizeof(float2)));
+    FIP_HIP(P->tw_m.ensure(twm.size() * sizeof(float2)));
+    FIP_HIP(P->wpost.ensure(wp.size() * sizeof(float2)));
+    FIP_HIP(P->corr.ensure(corr.size() * sizeof(float)));
+    FIP_HIP(hipMemcpy(P->tw_g.p, twg.data(), twg.size() * sizeof(float2), hipMemcpyHostToDevice));
+    FIP_HIP(hipMemcpy(P->tw_m.p, twm.data(), twm.size() * sizeof(float2), hipMemcpyHostToDevice));
+    FIP_HIP(hipMemcpy(P->wpost.p, wp.data(), wp.size() * sizeof(float2), hipMemcpyHostToDevice));
+    FIP_HIP(hipMemcpy(P->corr.p, corr.data(), corr.size() * sizeof(float), hipMemcpyHostToDevice));
+    P->t.tw_g = P->tw_g.as<float2>();
+    P->t.tw_m = P->tw_m.as<float2>();
+    P->t.wpost = P->wpost.as<float2>();
+    P->t.corr = P->corr.as<float>();
+    *out = P.get();
+    plans_[image] = std::move(P);
+    return FIP_OK;
+}
+
+Tuning Context::tuning(size_t image) const
+{
+    auto it = tunings_.find(image);
+    if (it != tunings_.end()) return it->second;
+    Tuning t;
+    t.stage1_chunk = 1 << 30;   // the large LDS band: one gridding pass
+    t.stage2_rows = 4;
+    t.splat_tile = 64;
+    t.persistent = 0;
+    return t;
+}
+
+int Context::begin_stage(int st, hipStream_t s)
+{
+    if (!profiling) return FIP_OK;
+    FIP_HIP(hipEventRecord(ev_[st][0], s));
+    return FIP_OK;
+}
+
+int Context::end_stage(int st, hipStream_t s)
+{
+    if (!profiling) return FIP_OK;
+    FIP_HIP(hipEventRecord(ev_[st][1], s));
+    ev_used_[st] = true;
+    return FIP_OK;
+}
+
+int Context::collect_stages()
+{
+    if (!profiling) return FIP_OK;
+    for (int st = 0; st < kStCount; ++st) {
+        if (!ev_used_[st]) continue;
+        FIP_HIP(hipEventSynchronize(ev_[st][1]));
+        float ms = 0.0f;
+        FIP_HIP(hipEventElapsedTime(&ms, ev_[st][0], ev_[st][1]));
+        stage_ms[st] += ms;
+        stage_n[st] += 1;
+        ev_used_[st] = false;
+    }
+    return FIP_OK;
+}
+
+// -------------------------------------------------------- launch sequence
+
+#define FIP_STAGE(C, st, s, call)                      \
+    do {                                               \
+        int rc_ = (C).begin_stage((st), (s));          \
+        if (rc_) return rc_;                           \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_error(e_, stage_name(st)); \
+        rc_ = (C).end_stage((st), (s));                \
+        if (rc_) return rc_;                           \
+    } while (0)
+
+int run_imaging(Context& C, const Plan& P, Work& W, const InputView& in, int S, int nmax, float cell,
+                const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st)
+{
+    const PlanTables& p = P.t;
+    const WorkView w = W.view(in.stride);
+    // r1r2_scale = cell * g (FIPkernels.cu:584), kept in FP64.
+    const double scale = (double)cell * (double)p.G;
+    FIP_STAGE(C, kStReset, st, launch_reset(in, w, p, S, cell, norm_dev, norm_const, st));
+    FIP_STAGE(C, kStBin, st, launch_bin(in, w, p, S, nmax, scale, st));
+    FIP_STAGE(C, kStScan, st, launch_scan(w, p, S, st));
+    FIP_STAGE(C, kStScatter, st, launch_scatter(w, p, S, nmax, st));
+    FIP_STAGE(C, kStGridFft, st, launch_stage1(w, p, S, tu, st));
+    FIP_STAGE(C, kStRowC2R, st, launch_stage2(w, p, S, tu, st));
+    FIP_STAGE(C, kStSplat, st, launch_splat(w, p, S, tu, st));
+    return FIP_OK;
+}
+
+int run_trigger(Context& C, const Plan& P, Work& W, size_t stride, const int* win_dev, int nwin, int tile,
+                float* out_dev, hipStream_t st)
+{
+    const WorkView w = W.view(stride);
+    FIP_STAGE(C, kStTrigger, st, launch_trigger(w, P.t, win_dev, nwin, tile, out_dev, st));
+    return FIP_OK;
+}
+
+int check_shape(size_t num_baselines, size_t image_size, size_t unit_size, float cell_size)
+{
+    if (num_baselines == 0) return set_error(FIP_ERR_ARG, "num_baselines must be positive");
+    if (num_baselines > (size_t)std::numeric_limits<int>::max() / 4)
+        return set_error(FIP_ERR_SIZE, "num_baselines too large");
+    if (!std::isfinite(cell_size)) return set_error(FIP_ERR_ARG, "cell_size must be finite");
+    if (!image_size_supported(image_size))
+        return set_error(FIP_ERR_SIZE,
+                         "image_size must be a multiple of 32 with grid 1.5*image = 3*2^k (64..8192), got " +
+                             std::to_string(image_size));
+    if (unit_size == 0 || unit_size > image_size)
+        return set_error(FIP_ERR_SIZE, "unit_size must be in [1, image_size]");
+    return FIP_OK;
+}
+
+}  // namespace fip

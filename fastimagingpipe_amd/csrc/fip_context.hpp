@@ -1,0 +1,98 @@
+// Device context: stream, per-size plan tables, work buffers and timing.
+#pragma once
+
+#include "fip_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace fip {
+
+// Records msg for fip_last_error() and returns code.
+int set_error(int code, const std::string& msg);
+int hip_error(hipError_t e, const char* what);
+
+#define FIP_HIP(expr)                                   \
+    do {                                                \
+        hipError_t fip_e_ = (expr);                     \
+        if (fip_e_ != hipSuccess) return hip_error(fip_e_, #expr); \
+    } while (0)
+
+// A grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+};
+
+struct Plan {
+    PlanTables t{};
+    DevBuf tw_g, tw_m, wpost, corr;
+};
+
+// Work buffers sized for S snapshots of one image size and stride.
+struct Work {
+    DevBuf rec, bkslot, sorted, hist, bstart, meta, geom, kt, pre, img, work;
+    WorkView view(size_t stride) const;
+    int ensure(int S, size_t stride, const PlanTables& p);
+    size_t bytes(int S, size_t stride, const PlanTables& p) const;
+    void release();
+};
+
+class Context {
+public:
+    static int get(Context** out);   // process-wide context on the current device
+
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+
+    int plan(size_t image, Plan** out);
+    Tuning tuning(size_t image) const;
+    void set_tuning(size_t image, const Tuning& t) { tunings_[image] = t; }
+
+    // Profiling: event pairs around each stage launch of the current call.
+    bool profiling = false;
+    double stage_ms[kStCount] = {};
+    long stage_n[kStCount] = {};
+    int begin_stage(int st, hipStream_t s);
+    int end_stage(int st, hipStream_t s);
+    int collect_stages();   // call after the stream has drained
+
+    Work work;
+    DevBuf in_vr, in_vi, in_b, in_v, out, win, nvis, norm;
+    PinnedBuf staging;
+
+private:
+    std::map<size_t, std::unique_ptr<Plan>> plans_;
+    std::map<size_t, Tuning> tunings_;
+    hipEvent_t ev_[kStCount][2] = {};
+    bool ev_used_[kStCount] = {};
+};
+
+// Enqueue reset..splat for S snapshots described by `in` into `w`.
+int run_imaging(Context& C, const Plan& P, Work& W, const InputView& in, int S, int nmax, float cell,
+                const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st);
+// Enqueue the trigger over windows `win_dev` (3 ints each) into out_dev.
+int run_trigger(Context& C, const Plan& P, Work& W, size_t stride, const int* win_dev, int nwin, int tile,
+                float* out_dev, hipStream_t st);
+
+// Shape checks shared by every entry point.
+int check_shape(size_t num_baselines, size_t image_size, size_t unit_size, float cell_size);
+
+}  // namespace fip

@@ -1,0 +1,192 @@
+// LDS-resident mixed-radix Stockham inverse FFT for gfx950.
+//
+// One transform of compile-time length N is held in a padded LDS array and
+// processed by NT threads (a "group"); several groups of one workgroup may
+// transform independent arrays in lock-step (every stage ends in a workgroup
+// barrier).  Each stage loads R elements per butterfly into registers,
+// applies the stage twiddles, runs an in-register radix-R butterfly and
+// writes back in Stockham (self-sorting) order, so no bit-reversal pass is
+// needed.  Twiddles come from a per-plan table generated on the host in FP64
+// and rounded once to FP32.
+//
+// Direction: inverse (unnormalised), X[k] = sum_j x[j] exp(+2 pi i j k / N),
+// the direction of the reference's cuFFT call (FIPkernels.cu:679, :741).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fip {
+
+// ---------------------------------------------------------------- complex ops
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (+i)
+__device__ __forceinline__ float2 cmul_i(float2 a) { return make_float2(-a.y, a.x); }
+// a * (-i)
+__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// LDS padding: one float2 of padding every 32 elements breaks the
+// power-of-two strides of the early Stockham stages across banks.
+__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 5) + 1; }
+
+// ---------------------------------------------------------- radix butterflies
+// In-register inverse DFTs of size R: v[s] <- sum_r v[r] exp(+2 pi i r s / R).
+template <int R> struct Bfly;
+
+template <> struct Bfly<2> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <> struct Bfly<3> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        const float s3 = 0.866025403784438646763723170752936183f;   // sqrt(3)/2
+        float2 t = cadd(v[1], v[2]);
+        float2 u = csub(v[1], v[2]);
+        float2 a = v[0];
+        v[0] = cadd(a, t);
+        float2 m = make_float2(a.x - 0.5f * t.x, a.y - 0.5f * t.y);
+        float2 iu = cmul_i(cscale(u, s3));
+        v[1] = cadd(m, iu);
+        v[2] = csub(m, iu);
+    }
+};
+
+template <> struct Bfly<4> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+        float2 c = cadd(v[1], v[3]), d = cmul_i(csub(v[1], v[3]));
+        v[0] = cadd(a, c);
+        v[2] = csub(a, c);
+        v[1] = cadd(b, d);
+        v[3] = csub(b, d);
+    }
+};
+
+template <> struct Bfly<8> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        const float r = 0.707106781186547524400844362104849039f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        Bfly<4>::run(e);
+        Bfly<4>::run(o);
+        // o[k] *= exp(+2 pi i k / 8)
+        float2 o1 = make_float2(r * (o[1].x - o[1].y), r * (o[1].x + o[1].y));
+        float2 o2 = cmul_i(o[2]);
+        float2 o3 = make_float2(-r * (o[3].x + o[3].y), r * (o[3].x - o[3].y));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+// ----------------------------------------------------------------- FFT plans
+// Radix lists for N = 3 * 2^k.  The host twiddle generator walks the same
+// list (fip_plan.cpp), so the two stay consistent by construction.
+template <int NT, int... Rs>
+struct PlanDef {
+    static constexpr int nt = NT;
+    static constexpr int nstages = sizeof...(Rs);
+    static constexpr int radix[sizeof...(Rs)] = {Rs...};
+};
+template <int N> struct FftPlan;
+#define FIP_PLAN(N_, NT_, ...)                                 \
+    template <> struct FftPlan<N_> : PlanDef<NT_, __VA_ARGS__> { \
+        static constexpr int n = N_;                           \
+    }
+FIP_PLAN(48, 64, 8, 2, 3);
+FIP_PLAN(96, 64, 8, 4, 3);
+FIP_PLAN(192, 64, 8, 8, 3);
+FIP_PLAN(384, 64, 8, 8, 2, 3);
+FIP_PLAN(768, 64, 8, 8, 4, 3);
+FIP_PLAN(1536, 64, 8, 8, 8, 3);
+FIP_PLAN(3072, 128, 8, 8, 8, 2, 3);
+FIP_PLAN(6144, 256, 8, 8, 8, 4, 3);
+FIP_PLAN(12288, 512, 8, 8, 8, 8, 3);
+#undef FIP_PLAN
+
+// One Stockham stage of radix R with Ns = product of earlier radices.
+// tw points at this stage's twiddles laid out [r-1][k], k < Ns.
+template <int N, int NT, int R, int Ns>
+__device__ __forceinline__ void fft_stage(float2* __restrict__ buf, const float2* __restrict__ tw, int t)
+{
+    constexpr int NB = N / R;                    // butterflies in this stage
+    constexpr int BPT = (NB + NT - 1) / NT;      // butterflies per thread
+    constexpr bool exact = (NB % NT) == 0;
+    float2 v[BPT][R];
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        const int j = t + b * NT;
+        if (exact || j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = buf[lds_pad(j + r * NB)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        const int j = t + b * NT;
+        if (exact || j < NB) {
+            const int k = j % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[(r - 1) * Ns + k]);
+            }
+            Bfly<R>::run(v[b]);
+            const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * Ns)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <class P, int S, int Ns, int TwOff, bool Done = (S >= P::nstages)>
+struct FftRun {
+    static __device__ __forceinline__ void run(float2* buf, const float2* tw, int t)
+    {
+        constexpr int R = P::radix[S];
+        fft_stage<P::n, P::nt, R, Ns>(buf, tw + TwOff, t);
+        constexpr int next_off = TwOff + (Ns > 1 ? (R - 1) * Ns : 0);
+        FftRun<P, S + 1, Ns * R, next_off>::run(buf, tw, t);
+    }
+};
+
+// Terminates the stage recursion once all radices are consumed.
+template <class P, int S, int Ns, int TwOff>
+struct FftRun<P, S, Ns, TwOff, true> {
+    static __device__ __forceinline__ void run(float2*, const float2*, int) {}
+};
+
+// Inverse FFT of the N-point padded LDS array buf.  Caller must have a
+// workgroup barrier between filling buf and this call; on return the result
+// is in buf (natural order) and visible to the whole workgroup.
+template <int N>
+__device__ __forceinline__ void fft_inverse(float2* buf, const float2* __restrict__ tw, int t)
+{
+    FftRun<FftPlan<N>, 0, 1, 0>::run(buf, tw, t);
+}
+
+// Number of twiddle entries the plan of length N uses (N - 1).
+__host__ __device__ constexpr int fft_twiddle_count(int n) { return n - 1; }
+
+}  // namespace fip

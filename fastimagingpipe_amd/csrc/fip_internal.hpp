@@ -1,0 +1,143 @@
+// Internal types and kernel launchers of the FIP-TOI MI355X path.
+// Not part of the public C ABI (include/fip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace fip {
+
+// ES gridding kernel shape parameter and support (FIPkernels.cu:198, 205).
+constexpr double kBeta = 15.3704324328;
+constexpr int kSupport = 8;
+// K tile height: stage 1 writes 16 consecutive image rows of one uv column as
+// one 128-byte line, stage 2 reads groups of rows from the same line.
+constexpr int kKTile = 16;
+
+// One binned visibility: integer floor and FP32 fraction of each grid
+// coordinate (computed from an FP64 product B * cell * g), and the weighted
+// visibility.  24 bytes.
+struct VisRec {
+    int b1;
+    float f1;
+    int b2;
+    float f2;
+    float vr;
+    float vi;
+};
+
+// Per-snapshot scalars produced and consumed on the device.
+struct SnapMeta {
+    int ngrid;            // visibilities with at least one grid tap
+    int rmax;             // largest |column tap|: stage 1 covers c = 0..rmax
+    int r1max;            // largest |row tap|: the int64 gridding band
+    unsigned maxv_bits;   // max(|re|,|im|) of weighted visibilities (float bits)
+    unsigned maxp_bits;   // max of the corrected image before re-projection
+    unsigned maxd_bits;   // max of the re-projected image (maxall entry)
+    int norm_n;           // N used by the splat's 1/N (num_baselines)
+    int nvis;             // visibilities present in this snapshot
+};
+
+// Per-snapshot re-projection geometry (FP64, from the V matrix).
+struct SnapGeom {
+    double m00, m01, m10, m11;   // affine: (a,b) = source - h  ->  (q1,q2) = target - h
+    double i00, i01, i10, i11;   // inverse of the affine part
+    double sx, sy;               // SIN shift per unit n' (pixels): -xi/cell, eta/cell
+    double delta;                // cell (rad)
+    double curv;                 // bound on the SIN map's curvature (1/px), <0: unbounded
+    int sin_on;                  // xi^2 + eta^2 != 0
+    int pad[3];
+};
+
+enum VisType { kVisF32 = 0, kVisBF16 = 1, kVisF16 = 2 };
+
+// Device-side view of the inputs of a batch of snapshots.
+struct InputView {
+    const void* vis_re;     // [s*stride + n] of VisType
+    const void* vis_im;
+    const float* bin;       // [(s*stride + n)*2 + {0,1}]
+    const float* vin;       // [s*9 + r*3 + c]
+    const int* nvis;        // optional per-snapshot counts (device); null: n_const
+    size_t stride;
+    int n_const;
+    int vis_type;
+};
+
+// Device tables for one image size.
+struct PlanTables {
+    int I, G, M;
+    float2* tw_g;      // stage twiddles for the length-G transform
+    float2* tw_m;      // stage twiddles for the length-M transform
+    float2* wpost;     // exp(+2 pi i k / G), k < M (C2R split)
+    float* corr;       // 1 / (c(t) * norm), t = 0..I/2
+};
+
+// Work buffers for a batch of up to S snapshots of one shape.
+struct WorkView {
+    VisRec* rec;       // [S*stride]
+    int2* bkslot;      // [S*stride]
+    VisRec* sorted;    // [S*stride]
+    int* hist;         // [S*G]
+    int* bstart;       // [S*(G+1)]
+    SnapMeta* meta;    // [S]
+    SnapGeom* geom;    // [S]
+    float2* kt;        // [S][I/16][M+1][16]
+    float* pre;        // [S][I][I] corrected image before re-projection
+    float* img;        // [S][I][I] re-projected images
+    unsigned* work;    // [1] persistent-kernel work counter
+    size_t stride;
+};
+
+// Tunable launch parameters (the autotuner picks among these).
+struct Tuning {
+    int stage1_chunk;   // int64 gridding rows held in LDS per pass
+    int stage2_rows;    // image rows per stage-2 workgroup (1, 2, 4)
+    int splat_tile;     // target tile edge of the splat (32 or 64)
+    int persistent;     // stage 1 as a persistent work-queue kernel
+};
+
+// Stage ids for timing.
+enum Stage {
+    kStReset = 0,
+    kStBin,
+    kStScan,
+    kStScatter,
+    kStGridFft,
+    kStRowC2R,
+    kStSplat,
+    kStTrigger,
+    kStCount
+};
+const char* stage_name(int st);
+
+// Kernel launchers (fip_kernels.hip).  All launch asynchronously on stream.
+hipError_t launch_reset(const InputView& in, const WorkView& w, const PlanTables& p, int S,
+                        float cell, const int* norm_n_host, int norm_n_const, hipStream_t st);
+hipError_t launch_bin(const InputView& in, const WorkView& w, const PlanTables& p, int S,
+                      int nmax, double scale, hipStream_t st);
+hipError_t launch_scan(const WorkView& w, const PlanTables& p, int S, hipStream_t st);
+hipError_t launch_scatter(const WorkView& w, const PlanTables& p, int S, int nmax, hipStream_t st);
+hipError_t launch_stage1(const WorkView& w, const PlanTables& p, int S, const Tuning& tu,
+                         hipStream_t st);
+hipError_t launch_stage2(const WorkView& w, const PlanTables& p, int S, const Tuning& tu,
+                         hipStream_t st);
+hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tuning& tu,
+                        hipStream_t st);
+// Trigger over nwin windows; win: device int[3*nwin] of snapshot slots in img.
+hipError_t launch_trigger(const WorkView& w, const PlanTables& p, const int* win, int nwin,
+                          int tile, float* out, hipStream_t st);
+
+// Inverse FFT of `count` independent length-n arrays in place (self-check).
+hipError_t launch_fft_check(int n, float2* data, const float2* tw, int count, hipStream_t st);
+
+// Sizes supported by the compiled FFT plans.
+bool fft_size_supported(int n);
+// Host generation of the stage twiddle table of the compiled plan for n
+// (FP64 -> FP32).  out must hold n-1 entries.
+void make_stage_twiddles(int n, float2* out);
+// Threads per transform of the compiled plan for n.
+int fft_threads(int n);
+
+}  // namespace fip

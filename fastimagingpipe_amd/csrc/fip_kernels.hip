@@ -1,0 +1,964 @@
+// HIP kernels of the FIP-TOI imaging-and-trigger path for gfx950 (CDNA4).
+//
+// Per batch of S snapshots the path runs eight launches (each covers all S
+// snapshots):
+//   reset    zero histograms / per-snapshot scalars, FP64 re-projection geometry
+//   bin      weighting (FIPkernels.cu:188-195), FP64 uv scaling, column-bucket
+//            histogram with per-visibility slots
+//   scan     exclusive scan of the bucket histogram
+//   scatter  visibilities into bucket order (counting sort by grid column)
+//   stage1   per uv column c >= 0: deterministic int64 LDS gridding of the
+//            Hermitian-folded grid column (FIPkernels.cu:197-245) and its
+//            length-g inverse FFT along the row axis, cropped and multiplied
+//            by the row correction (FIPkernels.cu:256-287, first axis)
+//   stage2   per image row: complex-to-real length-g inverse FFT along the
+//            column axis via a length-g/2 complex FFT, cropped, |.| and
+//            column correction -> corrected image (FIPkernels.cu:256-287)
+//   splat    per target tile: SIN/affine re-projection evaluated in FP64
+//            positions and the bilinear forward splat accumulated in int64
+//            LDS, plus the image maximum (FIPkernels.cu:289-497)
+//   trigger  per tile and window of three images: tLISI (FIPkernels.cu:415-552)
+//
+// The image identity the stages rely on: with the reference's modulation
+// (-1)^(r1+r2) in gridding and the (-1)^(|x|+|y|) flip after the FFT, the
+// cropped real part is D[y][x] = Re sum_{r1,r2} Gc[r1][r2] e^{+2 pi i (r1 y + r2 x)/g}
+// for the unmodulated grid Gc.  Folding Gc into the Hermitian grid
+// Gh[r] = (Gc[r] + conj(Gc[-r]))/2 makes D = sum Gh e^{...} exactly, so only
+// columns c = 0..g/2 of Gh are transformed and the second axis is a C2R.
+#include "fip_fft.hpp"
+#include "fip_internal.hpp"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+
+namespace fip {
+
+// ---------------------------------------------------------------- helpers
+
+// exp(beta*(sqrt(1-x^2)-1)) in the cancellation-free form
+// exp(-beta*x^2/(1+sqrt(1-x^2))) (FIPkernels.cu:67-71).
+__device__ __forceinline__ float es_f(float x)
+{
+    const float xx = x * x;
+    if (xx > 1.0f) return 0.0f;
+    const float s = sqrtf(1.0f - xx);
+    return expf(-(float)kBeta * xx / (1.0f + s));
+}
+
+template <class T> __device__ __forceinline__ float vis_to_f(const void* p, size_t i);
+template <> __device__ __forceinline__ float vis_to_f<float>(const void* p, size_t i)
+{
+    return static_cast<const float*>(p)[i];
+}
+template <> __device__ __forceinline__ float vis_to_f<__hip_bfloat16>(const void* p, size_t i)
+{
+    return __bfloat162float(static_cast<const __hip_bfloat16*>(p)[i]);
+}
+template <> __device__ __forceinline__ float vis_to_f<__half>(const void* p, size_t i)
+{
+    return __half2float(static_cast<const __half*>(p)[i]);
+}
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ unsigned wave_max_u(unsigned v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide max of a non-negative float (as ordered bits).  All threads call.
+template <int NT>
+__device__ __forceinline__ unsigned block_max_u(unsigned v, unsigned* sh)
+{
+    v = wave_max_u(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    unsigned r = 0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) r = max(r, sh[i]);
+    }
+    return r;   // valid in thread 0
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this maps the
+// hardware block id to a logical id so that each XCD works on one contiguous
+// run of logical blocks (neighbouring rows/tiles then share an L2).
+__device__ __forceinline__ int xcd_remap(int bid, int nb)
+{
+    constexpr int nx = 8;
+    const int per = nb / nx, rem = nb % nx;
+    const int x = bid % nx, k = bid / nx;
+    return x * per + min(x, rem) + k;
+}
+
+__device__ __forceinline__ long long to_fixed(float x)
+{
+    return (long long)rintf(x);   // |x| < 2^62 by construction; exact for |x| >= 2^24
+}
+
+// ----------------------------------------------------------------- reset
+
+__global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ nvis, int n_const,
+                        const int* __restrict__ norm_n, int norm_const, SnapMeta* meta, SnapGeom* geom,
+                        int* hist, unsigned* work, int S, int G, int I, float cell)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S * G) hist[i] = 0;
+    if (i == 0) work[0] = 0u;
+    if (i < S) {
+        SnapMeta m;
+        m.ngrid = 0;
+        m.rmax = -1;
+        m.r1max = -1;
+        m.maxv_bits = 0u;
+        m.maxp_bits = 0u;
+        m.maxd_bits = 0u;
+        m.nvis = nvis ? nvis[i] : n_const;
+        m.norm_n = norm_n ? norm_n[i] : norm_const;
+        meta[i] = m;
+
+        // Re-projection geometry (coordschange FIPkernels.cu:289-299 and the
+        // closed form of p2p :301-376), FP64 from the float V matrix.
+        const float* V = vin + (size_t)i * 9;
+        const double v00 = V[0], v01 = V[1], v10 = V[3], v11 = V[4];
+        const double v20 = V[6], v21 = V[7], v22 = V[8];
+        const double a22 = fabs(v22);
+        SnapGeom gm;
+        gm.m00 = -v00 / a22;
+        gm.m01 = v10 / a22;
+        gm.m10 = -v01 / a22;
+        gm.m11 = v11 / a22;
+        const double det = gm.m00 * gm.m11 - gm.m01 * gm.m10;
+        gm.i00 = gm.m11 / det;
+        gm.i01 = -gm.m01 / det;
+        gm.i10 = -gm.m10 / det;
+        gm.i11 = gm.m00 / det;
+        const double xi = v20 / v22, eta = v21 / v22;
+        const double delta = (double)cell;
+        gm.sx = -xi / delta;
+        gm.sy = eta / delta;
+        gm.delta = delta;
+        gm.sin_on = (xi * xi + eta * eta != 0.0) ? 1 : 0;
+        // Largest direction-cosine radius any source or target can reach.
+        const double h = (double)(I / 2);
+        const double nrm = sqrt(gm.m00 * gm.m00 + gm.m01 * gm.m01 + gm.m10 * gm.m10 + gm.m11 * gm.m11);
+        const double rmax = delta * (nrm * h * 1.4142135623730951 + 4.0 + h);
+        const double kap = sqrt(xi * xi + eta * eta) * rmax / sqrt(fmax(1e-300, 1.0 - rmax * rmax));
+        if (!(rmax < 0.9) || !(kap < 0.5) || !isfinite(det) || det == 0.0) {
+            gm.curv = -1.0;   // no usable bound: the splat scans whole images
+        } else {
+            const double onemk = 1.0 - kap;
+            gm.curv = sqrt(xi * xi + eta * eta) * delta / pow(1.0 - rmax * rmax, 1.5) /
+                      (onemk * onemk * onemk);
+        }
+        gm.pad[0] = gm.pad[1] = gm.pad[2] = 0;
+        geom[i] = gm;
+    }
+}
+
+// ------------------------------------------------------------------- bin
+
+template <class VT>
+__global__ __launch_bounds__(256) void k_bin(InputView in, VisRec* __restrict__ rec, int2* __restrict__ bs,
+                                             int* __restrict__ hist, SnapMeta* __restrict__ meta, int G,
+                                             double scale)
+{
+    __shared__ int sh_i[3][4];
+    __shared__ unsigned sh_u[4];
+    const int s = blockIdx.y;
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    const int ns = in.nvis ? in.nvis[s] : in.n_const;
+    const int M = G / 2;
+    int rmax = -1, r1max = -1, cnt = 0;
+    unsigned vmax = 0u;
+    if (n < ns) {
+        const size_t q = (size_t)s * in.stride + (size_t)n;
+        const float* V = in.vin + (size_t)s * 9;
+        // computeVisWeighted: v / |V00 V11 - V01 V10| (FIPkernels.cu:188-195)
+        const double J = fabs((double)V[0] * (double)V[4] - (double)V[1] * (double)V[3]);
+        const float vr = (float)((double)vis_to_f<VT>(in.vis_re, q) / J);
+        const float vi = (float)((double)vis_to_f<VT>(in.vis_im, q) / J);
+        // grid coordinates in FP64: B * (cell * g) (FIPkernels.cu:211-212, 584)
+        const double p1 = (double)in.bin[2 * q] * scale;
+        const double p2 = (double)in.bin[2 * q + 1] * scale;
+        const double fb1 = floor(p1), fb2 = floor(p2);
+        // Coordinates far outside the grid have no taps (and must not overflow int).
+        const bool inrange = fabs(p1) < (double)G && fabs(p2) < (double)G;
+        int2 key = make_int2(-1, 0);
+        if (inrange) {
+            VisRec r;
+            r.b1 = (int)fb1;
+            r.f1 = (float)(p1 - fb1);
+            r.b2 = (int)fb2;
+            r.f2 = (float)(p2 - fb2);
+            r.vr = vr;
+            r.vi = vi;
+            const int lo1 = max(r.b1 - 4 + (r.f1 > 0.0f ? 1 : 0), -M), hi1 = min(r.b1 + 4, M - 1);
+            const int lo2 = max(r.b2 - 4 + (r.f2 > 0.0f ? 1 : 0), -M), hi2 = min(r.b2 + 4, M - 1);
+            if (lo1 <= hi1 && lo2 <= hi2) {
+                const int bk = min(max(r.b2 + M, 0), G - 1);
+                const int slot = atomicAdd(&hist[(size_t)s * G + bk], 1);
+                key = make_int2(bk, slot);
+                rec[q] = r;
+                rmax = max(hi2, -lo2);
+                r1max = max(hi1, -lo1);
+                cnt = 1;
+                vmax = __float_as_uint(fmaxf(fabsf(vr), fabsf(vi)));
+            }
+        }
+        bs[q] = key;
+    }
+    rmax = wave_max_i(rmax);
+    r1max = wave_max_i(r1max);
+    cnt = wave_sum_i(cnt);
+    vmax = wave_max_u(vmax);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh_i[0][wid] = rmax;
+        sh_i[1][wid] = r1max;
+        sh_i[2][wid] = cnt;
+        sh_u[wid] = vmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) {
+            rmax = max(rmax, sh_i[0][k]);
+            r1max = max(r1max, sh_i[1][k]);
+            vmax = max(vmax, sh_u[k]);
+        }
+        cnt = sh_i[2][0] + sh_i[2][1] + sh_i[2][2] + sh_i[2][3];
+        rmax = max(rmax, sh_i[0][0]);
+        r1max = max(r1max, sh_i[1][0]);
+        vmax = max(vmax, sh_u[0]);
+        if (cnt > 0) {
+            atomicMax(&meta[s].rmax, rmax);
+            atomicMax(&meta[s].r1max, r1max);
+            atomicMax(&meta[s].maxv_bits, vmax);
+            atomicAdd(&meta[s].ngrid, cnt);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ scan
+
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ hist, int* __restrict__ bstart, int G)
+{
+    __shared__ int sh[1024];
+    const int s = blockIdx.x;
+    const int t = threadIdx.x;
+    const int per = (G + 1023) / 1024;
+    const int beg = min(t * per, G), end = min(beg + per, G);
+    const int* h = hist + (size_t)s * G;
+    int* o = bstart + (size_t)s * (G + 1);
+    int local = 0;
+    for (int i = beg; i < end; ++i) local += h[i];
+    sh[t] = local;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partial sums.
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = (t >= d) ? sh[t - d] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    int run = sh[t] - local;
+    for (int i = beg; i < end; ++i) {
+        o[i] = run;
+        run += h[i];
+    }
+    if (t == 1023) o[G] = sh[1023];
+}
+
+// --------------------------------------------------------------- scatter
+
+__global__ __launch_bounds__(256) void k_scatter(const VisRec* __restrict__ rec, const int2* __restrict__ bs,
+                                                 const int* __restrict__ bstart, VisRec* __restrict__ sorted,
+                                                 const SnapMeta* __restrict__ meta, size_t stride, int G)
+{
+    const int s = blockIdx.y;
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= meta[s].nvis) return;
+    const size_t q = (size_t)s * stride + (size_t)n;
+    const int2 key = bs[q];
+    if (key.x < 0) return;
+    sorted[(size_t)s * stride + (size_t)(bstart[(size_t)s * (G + 1) + key.x] + key.y)] = rec[q];
+}
+
+// ---------------------------------------------------------------- stage 1
+
+// Grid one visibility (or its Hermitian mirror) into uv column `col` for the
+// rows [lo, lo+CH) of the int64 band.  Taps follow FIPkernels.cu:213-243:
+// r in [ceil(p-4), floor(p+4)] clipped to [-g/2, g/2-1].
+__device__ __forceinline__ void grid_one(const VisRec& r, int col, bool mirror, int lo, int ch, int M,
+                                         float fscale, unsigned long long* are, unsigned long long* aim)
+{
+    const int lo2 = max(r.b2 - 4 + (r.f2 > 0.0f ? 1 : 0), -M);
+    const int hi2 = min(r.b2 + 4, M - 1);
+    if (col < lo2 || col > hi2) return;
+    const float w2 = es_f(((float)(col - r.b2) - r.f2) * 0.25f) * 0.5f * fscale;
+    const float vr = r.vr * w2;
+    const float vi = (mirror ? -r.vi : r.vi) * w2;
+    const int lo1 = max(r.b1 - 4 + (r.f1 > 0.0f ? 1 : 0), -M);
+    const int hi1 = min(r.b1 + 4, M - 1);
+    for (int t1 = lo1; t1 <= hi1; ++t1) {
+        const int row = mirror ? ((t1 == -M) ? -M : -t1) : t1;
+        const int q = row - lo;
+        if ((unsigned)q < (unsigned)ch) {
+            const float w1 = es_f(((float)(t1 - r.b1) - r.f1) * 0.25f);
+            atomicAdd(&are[q], (unsigned long long)to_fixed(vr * w1));
+            atomicAdd(&aim[q], (unsigned long long)to_fixed(vi * w1));
+        }
+    }
+}
+
+template <int G, int CH>
+__device__ __forceinline__ void stage1_column(int s, int c, const VisRec* __restrict__ sorted,
+                                              const int* __restrict__ bstart, const SnapMeta& m,
+                                              const float2* __restrict__ tw, const float* __restrict__ corr,
+                                              float2* __restrict__ kt, size_t stride, int I, float2* fb,
+                                              unsigned long long* are, unsigned long long* aim)
+{
+    constexpr int NT = FftPlan<G>::nt;
+    constexpr int M = G / 2;
+    const int t = threadIdx.x;
+    for (int i = t; i < G; i += NT) fb[lds_pad(i)] = make_float2(0.0f, 0.0f);
+
+    // Fixed-point scale: every cell sum is bounded by ngrid * maxv, so
+    // 2^k * ngrid * maxv < 2^61 keeps all int64 sums exact and in range.
+    const float maxv = __uint_as_float(m.maxv_bits);
+    int e = 0;
+    frexpf(maxv * (float)max(m.ngrid, 1), &e);
+    const float fscale = ldexpf(1.0f, 61 - e);
+    const float iscale = ldexpf(1.0f, e - 61);
+
+    const int* bst = bstart + (size_t)s * (G + 1);
+    const VisRec* vs = sorted + (size_t)s * stride;
+    const int cc = (c == M) ? -M : c;                        // direct column (centred)
+    const int cm = (c == 0) ? 0 : ((c == M) ? -M : -c);      // mirrored column
+    const int d0 = bst[min(max(cc - 4 + M, 0), G - 1)], d1 = bst[min(max(cc + 4 + M, 0), G - 1) + 1];
+    const int m0 = bst[min(max(cm - 4 + M, 0), G - 1)], m1 = bst[min(max(cm + 4 + M, 0), G - 1) + 1];
+    const int rlo = -m.r1max, rhi = min(m.r1max, M - 1);
+    for (int lo = rlo; lo <= rhi; lo += CH) {
+        for (int i = t; i < CH; i += NT) {
+            are[i] = 0ull;
+            aim[i] = 0ull;
+        }
+        __syncthreads();
+        for (int n = d0 + t; n < d1; n += NT) grid_one(vs[n], cc, false, lo, CH, M, fscale, are, aim);
+        for (int n = m0 + t; n < m1; n += NT) grid_one(vs[n], cm, true, lo, CH, M, fscale, are, aim);
+        __syncthreads();
+        const int nrow = min(CH, rhi - lo + 1);
+        for (int i = t; i < nrow; i += NT) {
+            const int row = lo + i;
+            const float re = (float)(long long)are[i] * iscale;
+            const float im = (float)(long long)aim[i] * iscale;
+            fb[lds_pad(row < 0 ? row + G : row)] = make_float2(re, im);
+        }
+        __syncthreads();
+    }
+    fft_inverse<G>(fb, tw, t);
+    // Crop y in [-h, h) and apply the row correction 1/(c(|y|) norm).
+    const int h = I / 2;
+    float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
+    for (int yp = t; yp < I; yp += NT) {
+        const int y = yp - h;
+        const float2 v = fb[lds_pad(y < 0 ? y + G : y)];
+        const float cy = corr[y < 0 ? -y : y];
+        ks[((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)] = make_float2(v.x * cy, v.y * cy);
+    }
+}
+
+template <int G, int CH>
+__global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1(const VisRec* __restrict__ sorted,
+                                                            const int* __restrict__ bstart,
+                                                            const SnapMeta* __restrict__ meta,
+                                                            const float2* __restrict__ tw,
+                                                            const float* __restrict__ corr,
+                                                            float2* __restrict__ kt, size_t stride, int I)
+{
+    __shared__ float2 fb[lds_len(G)];
+    __shared__ unsigned long long are[CH], aim[CH];
+    const int s = blockIdx.y;
+    const int c = blockIdx.x;
+    const SnapMeta m = meta[s];
+    if (c > m.rmax) return;
+    stage1_column<G, CH>(s, c, sorted, bstart, m, tw, corr, kt, stride, I, fb, are, aim);
+}
+
+// Persistent form: a fixed grid pulls (snapshot, column) items from a
+// counter until all S*(M+1) items are claimed; every workgroup exits.
+template <int G, int CH>
+__global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
+    const VisRec* __restrict__ sorted, const int* __restrict__ bstart, const SnapMeta* __restrict__ meta,
+    const float2* __restrict__ tw, const float* __restrict__ corr, float2* __restrict__ kt, size_t stride,
+    int I, int S, unsigned* work)
+{
+    __shared__ float2 fb[lds_len(G)];
+    __shared__ unsigned long long are[CH], aim[CH];
+    __shared__ unsigned item;
+    constexpr unsigned per = G / 2 + 1;
+    const unsigned total = (unsigned)S * per;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(work, 1u);
+        __syncthreads();
+        const unsigned it = item;
+        __syncthreads();
+        if (it >= total) break;
+        const int s = (int)(it / per), c = (int)(it % per);
+        const SnapMeta m = meta[s];
+        if (c > m.rmax) continue;
+        stage1_column<G, CH>(s, c, sorted, bstart, m, tw, corr, kt, stride, I, fb, are, aim);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- stage 2
+
+template <int M, int ROWS>
+__global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* __restrict__ kt,
+                                                                   SnapMeta* __restrict__ meta,
+                                                                   const float2* __restrict__ tw,
+                                                                   const float2* __restrict__ wpost,
+                                                                   const float* __restrict__ corr,
+                                                                   float* __restrict__ pre, int I)
+{
+    constexpr int NT = FftPlan<M>::nt;
+    constexpr int G = 2 * M;
+    constexpr int L = lds_len(M + 1);
+    __shared__ float2 zb[ROWS * L];
+    __shared__ unsigned shm[NT * ROWS / 64];
+    const int s = blockIdx.y;
+    const int y0 = xcd_remap(blockIdx.x, gridDim.x) * ROWS;
+    const int grp = threadIdx.x / NT, t = threadIdx.x % NT;
+    const int rmax = meta[s].rmax;
+    const int cmax = min(rmax, M);
+    float2* z = zb + grp * L;
+    const float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
+
+    // Gather K[y][c], c = 0..cmax, for the ROWS rows; zero the rest.
+    for (int c = cmax + 1 + t; c <= M; c += NT) z[lds_pad(c)] = make_float2(0.0f, 0.0f);
+    for (int L2 = threadIdx.x; L2 < (cmax + 1) * ROWS; L2 += NT * ROWS) {
+        const int c = L2 / ROWS, r = L2 % ROWS;
+        const int yp = y0 + r;
+        zb[r * L + lds_pad(c)] = ks[((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)];
+    }
+    __syncthreads();
+    // C2R split: Z[k] = E[k] + i O[k], E = K[k] + K~[k+M], O = (K[k] - K~[k+M]) e^{2 pi i k/G},
+    // K~[k+M] = conj(K[M-k]) (k > 0), K[M] (k = 0).  Pairs (p, M-p) are disjoint.
+    for (int p = t; p <= M / 2; p += NT) {
+        const float2 a = z[lds_pad(p)];
+        const float2 b = z[lds_pad(M - p)];
+        if (p == 0) {
+            z[lds_pad(0)] = cadd(cadd(a, b), cmul_i(csub(a, b)));
+        } else {
+            const float2 kp = cconj(b);   // K~[p+M]
+            const float2 zp = cadd(cadd(a, kp), cmul_i(cmul(csub(a, kp), wpost[p])));
+            if (p != M - p) {
+                const float2 km = cconj(a);   // K~[M-p+M]
+                const float2 zm = cadd(cadd(b, km), cmul_i(cmul(csub(b, km), wpost[M - p])));
+                z[lds_pad(M - p)] = zm;
+            }
+            z[lds_pad(p)] = zp;
+        }
+    }
+    __syncthreads();
+    fft_inverse<M>(z, tw, t);
+    // d[2n] = Re z[n], d[2n+1] = Im z[n]; crop x in [-h, h), |.| and the
+    // column correction (FIPkernels.cu:256-287).
+    const int h = I / 2;
+    float* out = pre + (size_t)s * I * I + (size_t)(y0 + grp) * I;
+    float lmax = 0.0f;
+    for (int col = t; col < I; col += NT) {
+        const int x = col - h;
+        const int mm = x < 0 ? x + G : x;
+        const float2 v = z[lds_pad(mm >> 1)];
+        const float d = (mm & 1) ? v.y : v.x;
+        const float val = fabsf(d) * corr[x < 0 ? -x : x];
+        out[col] = val;
+        lmax = fmaxf(lmax, val);
+    }
+    const unsigned bm = block_max_u<NT * ROWS>(__float_as_uint(lmax), shm);
+    if (threadIdx.x == 0) atomicMax(&meta[s].maxp_bits, bm);
+}
+
+// ----------------------------------------------------------------- splat
+
+__device__ __forceinline__ void sin_shift(const SnapGeom& g, double q1, double q2, double& o1, double& o2)
+{
+    o1 = q1;
+    o2 = q2;
+    if (!g.sin_on) return;
+    // l = -cell (P1 - c0), m = cell (P2 - c0), c0 = I/2 + 1 (FIPkernels.cu:315-316)
+    const float l = (float)(-g.delta * (q1 - 1.0));
+    const float mm = (float)(g.delta * (q2 - 1.0));
+    const float r2 = l * l + mm * mm;
+    if (r2 <= 1.0f) {
+        const float np = r2 / (1.0f + sqrtf(1.0f - r2));   // 1 - sqrt(1 - r^2)
+        o1 = q1 + g.sx * (double)np;
+        o2 = q2 + g.sy * (double)np;
+    }
+}
+
+template <int TS>
+__global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, SnapMeta* __restrict__ meta,
+                                               const SnapGeom* __restrict__ geom, float* __restrict__ img,
+                                               int I)
+{
+    constexpr int NT = 256;
+    __shared__ unsigned long long acc[TS * TS];
+    __shared__ int bb[4];
+    __shared__ unsigned shm[NT / 64];
+    const int s = blockIdx.y;
+    const int ntx = I / TS;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int R0 = (tile / ntx) * TS, C0 = (tile % ntx) * TS;
+    const SnapGeom g = geom[s];
+    const int h = I / 2;
+    for (int i = threadIdx.x; i < TS * TS; i += NT) acc[i] = 0ull;
+    if (threadIdx.x == 0) {
+        // Source bounding box: preimages of the corners of the open target
+        // box (C0-1, C0+TS) x (R0-1, R0+TS), padded by the curvature bound.
+        int i0 = 0, i1 = I - 1, j0 = 0, j1 = I - 1;
+        if (g.curv >= 0.0) {
+            double mini = 1e300, maxi = -1e300, minj = 1e300, maxj = -1e300;
+            for (int k = 0; k < 4; ++k) {
+                const double t1 = ((k & 1) ? (double)(C0 + TS) : (double)(C0 - 1)) - h;
+                const double t2 = ((k & 2) ? (double)(R0 + TS) : (double)(R0 - 1)) - h;
+                double q1 = t1, q2 = t2;
+                for (int it = 0; it < 8; ++it) {
+                    double o1, o2;
+                    sin_shift(g, q1, q2, o1, o2);
+                    q1 = t1 - (o1 - q1);
+                    q2 = t2 - (o2 - q2);
+                }
+                const double a = g.i00 * q1 + g.i01 * q2 + h;
+                const double b = g.i10 * q1 + g.i11 * q2 + h;
+                mini = fmin(mini, a);
+                maxi = fmax(maxi, a);
+                minj = fmin(minj, b);
+                maxj = fmax(maxj, b);
+            }
+            const double ext = (double)(TS + 2);
+            const int margin = 2 + (int)ceil(g.curv * ext * ext);
+            i0 = max(0, (int)floor(mini) - margin);
+            i1 = min(I - 1, (int)ceil(maxi) + margin);
+            j0 = max(0, (int)floor(minj) - margin);
+            j1 = min(I - 1, (int)ceil(maxj) + margin);
+        }
+        bb[0] = i0;
+        bb[1] = i1;
+        bb[2] = j0;
+        bb[3] = j1;
+    }
+    __syncthreads();
+    const int i0 = bb[0], i1 = bb[1], j0 = bb[2], j1 = bb[3];
+    const SnapMeta m = meta[s];
+    const float invn = 1.0f / (float)m.norm_n;   // FIPkernels.cu:391
+    int e = 0;
+    frexpf(__uint_as_float(m.maxp_bits) * invn * 16.0f, &e);
+    const float fscale = ldexpf(1.0f, 62 - e);
+    const float iscale = ldexpf(1.0f, e - 62);
+    const double lim = (double)(h - 1);
+    if (i1 >= i0 && j1 >= j0) {
+        const int wi = i1 - i0 + 1;
+        const int nsrc = wi * (j1 - j0 + 1);
+        const float* ps = pre + (size_t)s * I * I;
+        for (int idx = threadIdx.x; idx < nsrc; idx += NT) {
+            const int j = j0 + idx / wi;
+            const int i = i0 + idx % wi;
+            const float pv = ps[(size_t)j * I + i];
+            if (pv == 0.0f) continue;
+            const float val = pv * invn * fscale;
+            const double a = (double)(i - h), b = (double)(j - h);
+            double LL, MM;
+            sin_shift(g, g.m00 * a + g.m01 * b, g.m10 * a + g.m11 * b, LL, MM);
+            if (fabs(LL) < lim && fabs(MM) < lim) {
+                const double fl = floor(LL), fm = floor(MM);
+                const float wl1 = (float)(LL - fl), wm1 = (float)(MM - fm);
+                const float wl0 = 1.0f - wl1, wm0 = 1.0f - wm1;
+                const int r0 = (int)fm + h - R0, c0 = (int)fl + h - C0;
+                const int r1 = r0 + (wm1 > 0.0f ? 1 : 0), c1 = c0 + (wl1 > 0.0f ? 1 : 0);
+                // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL)
+                if ((unsigned)r0 < (unsigned)TS && (unsigned)c0 < (unsigned)TS)
+                    atomicAdd(&acc[r0 * TS + c0], (unsigned long long)to_fixed(wl0 * wm0 * val));
+                if ((unsigned)r1 < (unsigned)TS && (unsigned)c0 < (unsigned)TS)
+                    atomicAdd(&acc[r1 * TS + c0], (unsigned long long)to_fixed(wl0 * wm1 * val));
+                if ((unsigned)r0 < (unsigned)TS && (unsigned)c1 < (unsigned)TS)
+                    atomicAdd(&acc[r0 * TS + c1], (unsigned long long)to_fixed(wl1 * wm0 * val));
+                if ((unsigned)r1 < (unsigned)TS && (unsigned)c1 < (unsigned)TS)
+                    atomicAdd(&acc[r1 * TS + c1], (unsigned long long)to_fixed(wl1 * wm1 * val));
+            }
+        }
+    }
+    __syncthreads();
+    float* out = img + (size_t)s * I * I;
+    float lmax = 0.0f;
+    for (int q = threadIdx.x; q < TS * TS; q += NT) {
+        const float v = (float)(long long)acc[q] * iscale;
+        out[(size_t)(R0 + q / TS) * I + C0 + q % TS] = v;
+        lmax = fmaxf(lmax, v);
+    }
+    const unsigned bm = block_max_u<NT>(__float_as_uint(lmax), shm);
+    if (threadIdx.x == 0) atomicMax(&meta[s].maxd_bits, bm);
+}
+
+// --------------------------------------------------------------- trigger
+
+__global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, const SnapMeta* __restrict__ meta,
+                                                 const int* __restrict__ win, int I, int T,
+                                                 float* __restrict__ out)
+{
+    constexpr int NT = 256;
+    __shared__ float sh[3][NT / 64];
+    const int w = blockIdx.y;
+    const int un = I / T;
+    const int tile = blockIdx.x;
+    const int ti = tile / un, tj = tile % un;
+    const int s1 = win[3 * w], s2 = win[3 * w + 1], s3 = win[3 * w + 2];
+    const float* d1 = img + (size_t)s1 * I * I;
+    const float* d2 = img + (size_t)s2 * I * I;
+    const float* d3 = img + (size_t)s3 * I * I;
+    const float C = 1e-6f;   // FIPkernels.cu:811
+    float sum = 0.0f, mx = 0.0f, sr = 0.0f;
+    for (int p = threadIdx.x; p < T * T; p += NT) {
+        const size_t q = (size_t)(ti * T + p / T) * I + (size_t)(tj * T + p % T);
+        const float a = d1[q], b = d2[q], c = d3[q];
+        const float dd = fabsf(fabsf(a - b) - fabsf(b - c));
+        const float b2 = (b <= 0.0f) ? C : b;
+        sum += dd;
+        mx = fmaxf(mx, dd);
+        const float ratio = dd / b2;
+        sr += (ratio < 1.0f) ? ratio : 1.0f;
+    }
+    sum = wave_sum_f(sum);
+    mx = wave_max_f(mx);
+    sr = wave_sum_f(sr);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][wid] = sum;
+        sh[1][wid] = mx;
+        sh[2][wid] = sr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float S = 0.0f, MX = 0.0f, SR = 0.0f;
+        for (int k = 0; k < NT / 64; ++k) {
+            S += sh[0][k];
+            MX = fmaxf(MX, sh[1][k]);
+            SR += sh[2][k];
+        }
+        const float m3 = fmaxf(__uint_as_float(meta[s1].maxd_bits),
+                               fmaxf(__uint_as_float(meta[s2].maxd_bits), __uint_as_float(meta[s3].maxd_bits)));
+        const float Tf = (float)T;
+        // FIPkernels.cu:550
+        out[(size_t)w * un * un + tile] = 1.0f - (S / Tf / Tf) * MX * (SR / Tf / Tf) / m3 / m3;
+    }
+}
+
+// ------------------------------------------------------------- launchers
+
+const char* stage_name(int st)
+{
+    static const char* names[kStCount] = {"reset", "bin", "scan", "scatter",
+                                          "grid_colfft", "row_c2r", "splat", "trigger"};
+    return (st >= 0 && st < kStCount) ? names[st] : "?";
+}
+
+hipError_t launch_reset(const InputView& in, const WorkView& w, const PlanTables& p, int S, float cell,
+                        const int* norm_n_dev, int norm_n_const, hipStream_t st)
+{
+    const int total = max(S * p.G, S);
+    hipLaunchKernelGGL(k_reset, dim3((total + 255) / 256), dim3(256), 0, st, in.vin, in.nvis, in.n_const,
+                       norm_n_dev, norm_n_const, w.meta, w.geom, w.hist, w.work, S, p.G, p.I, cell);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin(const InputView& in, const WorkView& w, const PlanTables& p, int S, int nmax,
+                      double scale, hipStream_t st)
+{
+    if (nmax <= 0) return hipSuccess;
+    dim3 grid((nmax + 255) / 256, S);
+    switch (in.vis_type) {
+    case kVisF32:
+        hipLaunchKernelGGL(k_bin<float>, grid, dim3(256), 0, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G, scale);
+        break;
+    case kVisBF16:
+        hipLaunchKernelGGL(k_bin<__hip_bfloat16>, grid, dim3(256), 0, st, in, w.rec, w.bkslot, w.hist, w.meta,
+                           p.G, scale);
+        break;
+    case kVisF16:
+        hipLaunchKernelGGL(k_bin<__half>, grid, dim3(256), 0, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G,
+                           scale);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const WorkView& w, const PlanTables& p, int S, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scan, dim3(S), dim3(1024), 0, st, w.hist, w.bstart, p.G);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const WorkView& w, const PlanTables& p, int S, int nmax, hipStream_t st)
+{
+    if (nmax <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3((nmax + 255) / 256, S), dim3(256), 0, st, w.rec, w.bkslot, w.bstart,
+                       w.sorted, w.meta, w.stride, p.G);
+    return hipGetLastError();
+}
+
+template <int G, int CH>
+static void stage1_dispatch(const WorkView& w, const PlanTables& p, int S, bool persistent, hipStream_t st)
+{
+    constexpr int NT = FftPlan<G>::nt;
+    if (persistent) {
+        int dev = 0, ncu = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage1_persistent<G, CH>, NT, 0);
+        const int nb = max(1, ncu * max(per_cu, 1));
+        hipLaunchKernelGGL((k_stage1_persistent<G, CH>), dim3(nb), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
+                           p.tw_g, p.corr, w.kt, w.stride, p.I, S, w.work);
+    } else {
+        hipLaunchKernelGGL((k_stage1<G, CH>), dim3(G / 2 + 1, S), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
+                           p.tw_g, p.corr, w.kt, w.stride, p.I);
+    }
+}
+
+// LDS int64 band (rows per gridding pass) for each transform length; the
+// second value is the small-chunk variant that lets two workgroups share a CU.
+template <int G> struct ChunkOpts;
+template <> struct ChunkOpts<96> { static constexpr int big = 96, small = 48; };
+template <> struct ChunkOpts<192> { static constexpr int big = 192, small = 96; };
+template <> struct ChunkOpts<384> { static constexpr int big = 384, small = 192; };
+template <> struct ChunkOpts<768> { static constexpr int big = 768, small = 384; };
+template <> struct ChunkOpts<1536> { static constexpr int big = 1536, small = 768; };
+template <> struct ChunkOpts<3072> { static constexpr int big = 3072, small = 1536; };
+template <> struct ChunkOpts<6144> { static constexpr int big = 3584, small = 1792; };
+template <> struct ChunkOpts<12288> { static constexpr int big = 3072, small = 1536; };
+
+template <int G>
+static void stage1_pick(const WorkView& w, const PlanTables& p, int S, const Tuning& tu, hipStream_t st)
+{
+    if (tu.stage1_chunk <= ChunkOpts<G>::small)
+        stage1_dispatch<G, ChunkOpts<G>::small>(w, p, S, tu.persistent != 0, st);
+    else
+        stage1_dispatch<G, ChunkOpts<G>::big>(w, p, S, tu.persistent != 0, st);
+}
+
+hipError_t launch_stage1(const WorkView& w, const PlanTables& p, int S, const Tuning& tu, hipStream_t st)
+{
+    switch (p.G) {
+    case 96: stage1_pick<96>(w, p, S, tu, st); break;
+    case 192: stage1_pick<192>(w, p, S, tu, st); break;
+    case 384: stage1_pick<384>(w, p, S, tu, st); break;
+    case 768: stage1_pick<768>(w, p, S, tu, st); break;
+    case 1536: stage1_pick<1536>(w, p, S, tu, st); break;
+    case 3072: stage1_pick<3072>(w, p, S, tu, st); break;
+    case 6144: stage1_pick<6144>(w, p, S, tu, st); break;
+    case 12288: stage1_pick<12288>(w, p, S, tu, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int M, int ROWS>
+static void stage2_dispatch(const WorkView& w, const PlanTables& p, int S, hipStream_t st)
+{
+    constexpr int NT = FftPlan<M>::nt * ROWS;
+    hipLaunchKernelGGL((k_stage2<M, ROWS>), dim3(p.I / ROWS, S), dim3(NT), 0, st, w.kt, w.meta, p.tw_m, p.wpost,
+                       p.corr, w.pre, p.I);
+}
+
+template <int M>
+static void stage2_pick(const WorkView& w, const PlanTables& p, int S, const Tuning& tu, hipStream_t st)
+{
+    // LDS per row is (M + M/32 + 2) * 8 bytes; keep a workgroup under 128 KiB.
+    constexpr int row_bytes = lds_len(M + 1) * 8;
+    if constexpr (4 * row_bytes <= 128 * 1024) {
+        if (tu.stage2_rows >= 4) {
+            stage2_dispatch<M, 4>(w, p, S, st);
+            return;
+        }
+    }
+    if constexpr (2 * row_bytes <= 128 * 1024) {
+        if (tu.stage2_rows >= 2) {
+            stage2_dispatch<M, 2>(w, p, S, st);
+            return;
+        }
+    }
+    stage2_dispatch<M, 1>(w, p, S, st);
+}
+
+hipError_t launch_stage2(const WorkView& w, const PlanTables& p, int S, const Tuning& tu, hipStream_t st)
+{
+    switch (p.M) {
+    case 48: stage2_pick<48>(w, p, S, tu, st); break;
+    case 96: stage2_pick<96>(w, p, S, tu, st); break;
+    case 192: stage2_pick<192>(w, p, S, tu, st); break;
+    case 384: stage2_pick<384>(w, p, S, tu, st); break;
+    case 768: stage2_pick<768>(w, p, S, tu, st); break;
+    case 1536: stage2_pick<1536>(w, p, S, tu, st); break;
+    case 3072: stage2_pick<3072>(w, p, S, tu, st); break;
+    case 6144: stage2_pick<6144>(w, p, S, tu, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tuning& tu, hipStream_t st)
+{
+    if (tu.splat_tile >= 64 && p.I % 64 == 0) {
+        const int nt = (p.I / 64) * (p.I / 64);
+        hipLaunchKernelGGL(k_splat<64>, dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.img, p.I);
+    } else {
+        const int nt = (p.I / 32) * (p.I / 32);
+        hipLaunchKernelGGL(k_splat<32>, dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.img, p.I);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trigger(const WorkView& w, const PlanTables& p, const int* win, int nwin, int tile, float* out,
+                          hipStream_t st)
+{
+    const int un = p.I / tile;
+    if (un <= 0 || nwin <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trigger, dim3(un * un, nwin), dim3(256), 0, st, w.img, w.meta, win, p.I, tile, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ FFT check
+
+template <int N>
+__global__ __launch_bounds__(FftPlan<N>::nt) void k_fft_check(float2* __restrict__ data, const float2* __restrict__ tw)
+{
+    constexpr int NT = FftPlan<N>::nt;
+    __shared__ float2 fb[lds_len(N)];
+    float2* d = data + (size_t)blockIdx.x * N;
+    for (int i = threadIdx.x; i < N; i += NT) fb[lds_pad(i)] = d[i];
+    __syncthreads();
+    fft_inverse<N>(fb, tw, threadIdx.x);
+    for (int i = threadIdx.x; i < N; i += NT) d[i] = fb[lds_pad(i)];
+}
+
+hipError_t launch_fft_check(int n, float2* data, const float2* tw, int count, hipStream_t st)
+{
+    switch (n) {
+#define FIP_FFT_CASE(N_)                                                                                  \
+    case N_:                                                                                              \
+        hipLaunchKernelGGL(k_fft_check<N_>, dim3(count), dim3(FftPlan<N_>::nt), 0, st, data, tw);        \
+        break;
+        FIP_FFT_CASE(48)
+        FIP_FFT_CASE(96)
+        FIP_FFT_CASE(192)
+        FIP_FFT_CASE(384)
+        FIP_FFT_CASE(768)
+        FIP_FFT_CASE(1536)
+        FIP_FFT_CASE(3072)
+        FIP_FFT_CASE(6144)
+        FIP_FFT_CASE(12288)
+#undef FIP_FFT_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------- plan-side helpers
+
+template <int N>
+static void twiddles_for(float2* out)
+{
+    using P = FftPlan<N>;
+    int ns = 1, off = 0;
+    for (int s = 0; s < P::nstages; ++s) {
+        const int R = P::radix[s];
+        if (ns > 1) {
+            for (int r = 1; r < R; ++r) {
+                for (int k = 0; k < ns; ++k) {
+                    // exp(+2 pi i r k / (ns R)), argument reduced exactly in integers
+                    const long long num = (long long)r * k % ((long long)ns * R);
+                    const double ang = 2.0 * M_PI * (double)num / (double)(ns * R);
+                    out[off + (r - 1) * ns + k] = make_float2((float)cos(ang), (float)sin(ang));
+                }
+            }
+            off += (R - 1) * ns;
+        }
+        ns *= R;
+    }
+}
+
+bool fft_size_supported(int n)
+{
+    switch (n) {
+    case 48: case 96: case 192: case 384: case 768: case 1536: case 3072: case 6144: case 12288:
+        return true;
+    default:
+        return false;
+    }
+}
+
+void make_stage_twiddles(int n, float2* out)
+{
+    switch (n) {
+    case 48: twiddles_for<48>(out); break;
+    case 96: twiddles_for<96>(out); break;
+    case 192: twiddles_for<192>(out); break;
+    case 384: twiddles_for<384>(out); break;
+    case 768: twiddles_for<768>(out); break;
+    case 1536: twiddles_for<1536>(out); break;
+    case 3072: twiddles_for<3072>(out); break;
+    case 6144: twiddles_for<6144>(out); break;
+    case 12288: twiddles_for<12288>(out); break;
+    default: break;
+    }
+}
+
+int fft_threads(int n)
+{
+    switch (n) {
+    case 48: return FftPlan<48>::nt;
+    case 96: return FftPlan<96>::nt;
+    case 192: return FftPlan<192>::nt;
+    case 384: return FftPlan<384>::nt;
+    case 768: return FftPlan<768>::nt;
+    case 1536: return FftPlan<1536>::nt;
+    case 3072: return FftPlan<3072>::nt;
+    case 6144: return FftPlan<6144>::nt;
+    case 12288: return FftPlan<12288>::nt;
+    default: return 0;
+    }
+}
+
+}  // namespace fip

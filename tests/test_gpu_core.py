@@ -1,0 +1,66 @@
+"""GPU parity of the core path against the FP64 oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+from tests.inputs import rel_rms, snapshots
+
+pytestmark = pytest.mark.gpu
+
+FFT_SIZES = [48, 96, 192, 384, 768, 1536, 3072, 6144, 12288]
+
+
+@pytest.mark.parametrize("n", FFT_SIZES)
+def test_lds_fft_matches_numpy(fip, n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))).astype(np.complex64)
+    got = fip.debug_fft(x)
+    ref = np.fft.ifft(x.astype(np.complex128), axis=1) * n
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 2e-6 * np.log2(n), err
+
+
+@pytest.mark.parametrize("image,nbl,cell", [(64, 300, 2e-4), (128, 900, 1e-4), (256, 2016, 5.8e-5)])
+def test_image_matches_oracle(fip, oracle, image, nbl, cell):
+    vr, vi, B, V = snapshots(11 + image, 3, nbl, image, cell)
+    pre, img, mx = fip.image(vr, vi, B, V, nbl, image, 3, cell)
+    for s in range(3):
+        opre, oimg, omax = oracle.image(vr[s], vi[s], B[s], V[s], image, cell, want_pre=True)
+        e_pre = rel_rms(pre[s], opre)
+        e_img = rel_rms(img[s], oimg)
+        print(f"I={image} s={s} pre rel-rms {e_pre:.2e} img rel-rms {e_img:.2e} max {mx[s]:.6g}/{omax:.6g}")
+        assert e_pre < 2e-5
+        assert e_img < 5e-5
+        assert abs(mx[s] - omax) <= 1e-4 * omax
+
+
+def test_pipe_matches_oracle_small(fip, oracle):
+    image, nbl, cell, tile = 256, 2016, 5.8e-5, 16
+    vr, vi, B, V = snapshots(5, 3, nbl, image, cell)
+    got = fip.fipipe(vr, vi, B, V, nbl, image, 3, cell, tile)
+    ref = oracle.pipe(vr, vi, B, V, nbl, image, 3, cell, tile)
+    err = np.abs(got - ref).max()
+    print("tLISI max abs err", err, "min tLISI", ref.min())
+    assert err < 2e-5
+
+
+def test_pipe_deterministic(fip):
+    image, nbl, cell, tile = 256, 2016, 5.8e-5, 16
+    vr, vi, B, V = snapshots(9, 3, nbl, image, cell)
+    a = fip.fipipe(vr, vi, B, V, nbl, image, 3, cell, tile)
+    b = fip.fipipe(vr, vi, B, V, nbl, image, 3, cell, tile)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    p1, i1, m1 = fip.image(vr, vi, B, V, nbl, image, 3, cell)
+    p2, i2, m2 = fip.image(vr, vi, B, V, nbl, image, 3, cell)
+    assert np.array_equal(p1.view(np.uint32), p2.view(np.uint32))
+    assert np.array_equal(i1.view(np.uint32), i2.view(np.uint32))
+
+
+def test_trigger_matches_oracle(fip, oracle):
+    rng = np.random.default_rng(3)
+    I, T = 256, 16
+    d = rng.uniform(0, 1, (3, I, I)).astype(np.float32)
+    d[1, :5, :7] = 0.0   # exercise the d2 <= 0 guard
+    mx = d.reshape(3, -1).max(axis=1).astype(np.float32)
+    got = fip.trigger(d[0], d[1], d[2], mx, T)
+    ref = oracle.trigger(d[0], d[1], d[2], mx.astype(np.float64), T)
+    assert np.abs(got - ref).max() < 1e-5
