@@ -47,8 +47,10 @@ struct SnapGeom {
     double sx, sy;               // SIN shift per unit n' (pixels): -xi/cell, eta/cell
     double delta;                // cell (rad)
     double curv;                 // bound on the SIN map's curvature (1/px), <0: unbounded
+    double kappa;                // bound on |grad| of the SIN shift over the image
     int sin_on;                  // xi^2 + eta^2 != 0
-    int pad[3];
+    int phased;                  // sources 3 apart map >= 2*sqrt(2) apart: phased float splat is exact-order
+    int pad[2];
 };
 
 enum VisType { kVisF32 = 0, kVisBF16 = 1, kVisF16 = 2 };

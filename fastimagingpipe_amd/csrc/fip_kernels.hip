@@ -167,74 +167,104 @@ __global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ n
         gm.sy = eta / delta;
         gm.delta = delta;
         gm.sin_on = (xi * xi + eta * eta != 0.0) ? 1 : 0;
-        // Largest direction-cosine radius any source or target can reach.
+        // Singular values of the affine part, and the largest direction-cosine
+        // radius any source or target can reach.
         const double h = (double)(I / 2);
-        const double nrm = sqrt(gm.m00 * gm.m00 + gm.m01 * gm.m01 + gm.m10 * gm.m10 + gm.m11 * gm.m11);
-        const double rmax = delta * (nrm * h * 1.4142135623730951 + 4.0 + h);
+        const double tr = gm.m00 * gm.m00 + gm.m01 * gm.m01 + gm.m10 * gm.m10 + gm.m11 * gm.m11;
+        const double dd = fabs(det);
+        const double disc = sqrt(fmax(0.0, tr * tr - 4.0 * dd * dd));
+        const double smax = sqrt(0.5 * (tr + disc)), smin = sqrt(fmax(0.0, 0.5 * (tr - disc)));
+        const double rmax = delta * (smax * h * 1.4142135623730951 + 4.0 + h);
         const double kap = sqrt(xi * xi + eta * eta) * rmax / sqrt(fmax(1e-300, 1.0 - rmax * rmax));
+        gm.kappa = kap;
         if (!(rmax < 0.9) || !(kap < 0.5) || !isfinite(det) || det == 0.0) {
             gm.curv = -1.0;   // no usable bound: the splat scans whole images
+            gm.phased = 0;
         } else {
-            const double onemk = 1.0 - kap;
-            gm.curv = sqrt(xi * xi + eta * eta) * delta / pow(1.0 - rmax * rmax, 1.5) /
-                      (onemk * onemk * onemk);
+            gm.curv = kap;
+            // Two sources 3 apart map at least 3 (smin - kappa smax) apart; at
+            // >= 2 sqrt(2) their bilinear footprints are disjoint, so the nine
+            // source classes (i mod 3, j mod 3) can be splatted phase by phase.
+            gm.phased = (3.0 * (smin - kap * smax) >= 2.8284271247461903 + 0.02) ? 1 : 0;
         }
-        gm.pad[0] = gm.pad[1] = gm.pad[2] = 0;
+        gm.pad[0] = gm.pad[1] = 0;
         geom[i] = gm;
     }
 }
 
 // ------------------------------------------------------------------- bin
 
+// Visibilities per thread of the bin kernel (a block bins 256 * kBinVpt).
+constexpr int kBinVpt = 8;
+
+// Block-privatised column histogram: slots within the block come from LDS
+// atomics, then one global atomic per non-empty bucket reserves the block's
+// range, so the hot central buckets see one global atomic per block instead
+// of one per visibility.
 template <class VT>
 __global__ __launch_bounds__(256) void k_bin(InputView in, VisRec* __restrict__ rec, int2* __restrict__ bs,
                                              int* __restrict__ hist, SnapMeta* __restrict__ meta, int G,
                                              double scale)
 {
+    extern __shared__ int lh[];   // G ints: local counts, then block bases
     __shared__ int sh_i[3][4];
     __shared__ unsigned sh_u[4];
     const int s = blockIdx.y;
-    const int n = blockIdx.x * 256 + threadIdx.x;
     const int ns = in.nvis ? in.nvis[s] : in.n_const;
     const int M = G / 2;
+    for (int b = threadIdx.x; b < G; b += 256) lh[b] = 0;
+    __syncthreads();
     int rmax = -1, r1max = -1, cnt = 0;
     unsigned vmax = 0u;
-    if (n < ns) {
+    int bk[kBinVpt], slot[kBinVpt];
+    const float* V = in.vin + (size_t)s * 9;
+    // computeVisWeighted: v / |V00 V11 - V01 V10| (FIPkernels.cu:188-195)
+    const double J = fabs((double)V[0] * (double)V[4] - (double)V[1] * (double)V[3]);
+#pragma unroll
+    for (int k = 0; k < kBinVpt; ++k) {
+        bk[k] = -1;
+        slot[k] = 0;
+        const int n = blockIdx.x * (256 * kBinVpt) + k * 256 + threadIdx.x;
+        if (n >= ns) continue;
         const size_t q = (size_t)s * in.stride + (size_t)n;
-        const float* V = in.vin + (size_t)s * 9;
-        // computeVisWeighted: v / |V00 V11 - V01 V10| (FIPkernels.cu:188-195)
-        const double J = fabs((double)V[0] * (double)V[4] - (double)V[1] * (double)V[3]);
         const float vr = (float)((double)vis_to_f<VT>(in.vis_re, q) / J);
         const float vi = (float)((double)vis_to_f<VT>(in.vis_im, q) / J);
         // grid coordinates in FP64: B * (cell * g) (FIPkernels.cu:211-212, 584)
         const double p1 = (double)in.bin[2 * q] * scale;
         const double p2 = (double)in.bin[2 * q + 1] * scale;
-        const double fb1 = floor(p1), fb2 = floor(p2);
         // Coordinates far outside the grid have no taps (and must not overflow int).
-        const bool inrange = fabs(p1) < (double)G && fabs(p2) < (double)G;
-        int2 key = make_int2(-1, 0);
-        if (inrange) {
-            VisRec r;
-            r.b1 = (int)fb1;
-            r.f1 = (float)(p1 - fb1);
-            r.b2 = (int)fb2;
-            r.f2 = (float)(p2 - fb2);
-            r.vr = vr;
-            r.vi = vi;
-            const int lo1 = max(r.b1 - 4 + (r.f1 > 0.0f ? 1 : 0), -M), hi1 = min(r.b1 + 4, M - 1);
-            const int lo2 = max(r.b2 - 4 + (r.f2 > 0.0f ? 1 : 0), -M), hi2 = min(r.b2 + 4, M - 1);
-            if (lo1 <= hi1 && lo2 <= hi2) {
-                const int bk = min(max(r.b2 + M, 0), G - 1);
-                const int slot = atomicAdd(&hist[(size_t)s * G + bk], 1);
-                key = make_int2(bk, slot);
-                rec[q] = r;
-                rmax = max(hi2, -lo2);
-                r1max = max(hi1, -lo1);
-                cnt = 1;
-                vmax = __float_as_uint(fmaxf(fabsf(vr), fabsf(vi)));
-            }
-        }
-        bs[q] = key;
+        if (!(fabs(p1) < (double)G && fabs(p2) < (double)G)) continue;
+        const double fb1 = floor(p1), fb2 = floor(p2);
+        VisRec r;
+        r.b1 = (int)fb1;
+        r.f1 = (float)(p1 - fb1);
+        r.b2 = (int)fb2;
+        r.f2 = (float)(p2 - fb2);
+        r.vr = vr;
+        r.vi = vi;
+        const int lo1 = max(r.b1 - 4 + (r.f1 > 0.0f ? 1 : 0), -M), hi1 = min(r.b1 + 4, M - 1);
+        const int lo2 = max(r.b2 - 4 + (r.f2 > 0.0f ? 1 : 0), -M), hi2 = min(r.b2 + 4, M - 1);
+        if (lo1 > hi1 || lo2 > hi2) continue;
+        bk[k] = min(max(r.b2 + M, 0), G - 1);
+        slot[k] = atomicAdd(&lh[bk[k]], 1);
+        rec[q] = r;
+        rmax = max(rmax, max(hi2, -lo2));
+        r1max = max(r1max, max(hi1, -lo1));
+        cnt += 1;
+        vmax = max(vmax, __float_as_uint(fmaxf(fabsf(vr), fabsf(vi))));
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < G; b += 256) {
+        const int c = lh[b];
+        if (c > 0) lh[b] = atomicAdd(&hist[(size_t)s * G + b], c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBinVpt; ++k) {
+        const int n = blockIdx.x * (256 * kBinVpt) + k * 256 + threadIdx.x;
+        if (n >= ns) continue;
+        const size_t q = (size_t)s * in.stride + (size_t)n;
+        bs[q] = bk[k] >= 0 ? make_int2(bk[k], lh[bk[k]] + slot[k]) : make_int2(-1, 0);
     }
     rmax = wave_max_i(rmax);
     r1max = wave_max_i(r1max);
@@ -253,11 +283,8 @@ __global__ __launch_bounds__(256) void k_bin(InputView in, VisRec* __restrict__ 
             rmax = max(rmax, sh_i[0][k]);
             r1max = max(r1max, sh_i[1][k]);
             vmax = max(vmax, sh_u[k]);
+            cnt += sh_i[2][k];
         }
-        cnt = sh_i[2][0] + sh_i[2][1] + sh_i[2][2] + sh_i[2][3];
-        rmax = max(rmax, sh_i[0][0]);
-        r1max = max(r1max, sh_i[1][0]);
-        vmax = max(vmax, sh_u[0]);
         if (cnt > 0) {
             atomicMax(&meta[s].rmax, rmax);
             atomicMax(&meta[s].r1max, r1max);
@@ -318,7 +345,7 @@ __global__ __launch_bounds__(256) void k_scatter(const VisRec* __restrict__ rec,
 // rows [lo, lo+CH) of the int64 band.  Taps follow FIPkernels.cu:213-243:
 // r in [ceil(p-4), floor(p+4)] clipped to [-g/2, g/2-1].
 __device__ __forceinline__ void grid_one(const VisRec& r, int col, bool mirror, int lo, int ch, int M,
-                                         float fscale, unsigned long long* are, unsigned long long* aim)
+                                         float fscale, double* are, double* aim)
 {
     const int lo2 = max(r.b2 - 4 + (r.f2 > 0.0f ? 1 : 0), -M);
     const int hi2 = min(r.b2 + 4, M - 1);
@@ -333,8 +360,8 @@ __device__ __forceinline__ void grid_one(const VisRec& r, int col, bool mirror, 
         const int q = row - lo;
         if ((unsigned)q < (unsigned)ch) {
             const float w1 = es_f(((float)(t1 - r.b1) - r.f1) * 0.25f);
-            atomicAdd(&are[q], (unsigned long long)to_fixed(vr * w1));
-            atomicAdd(&aim[q], (unsigned long long)to_fixed(vi * w1));
+            atomicAdd(&are[q], rint((double)(vr * w1)));
+            atomicAdd(&aim[q], rint((double)(vi * w1)));
         }
     }
 }
@@ -344,20 +371,22 @@ __device__ __forceinline__ void stage1_column(int s, int c, const VisRec* __rest
                                               const int* __restrict__ bstart, const SnapMeta& m,
                                               const float2* __restrict__ tw, const float* __restrict__ corr,
                                               float2* __restrict__ kt, size_t stride, int I, float2* fb,
-                                              unsigned long long* are, unsigned long long* aim)
+                                              double* are, double* aim)
 {
     constexpr int NT = FftPlan<G>::nt;
     constexpr int M = G / 2;
     const int t = threadIdx.x;
     for (int i = t; i < G; i += NT) fb[lds_pad(i)] = make_float2(0.0f, 0.0f);
 
-    // Fixed-point scale: every cell sum is bounded by ngrid * maxv, so
-    // 2^k * ngrid * maxv < 2^61 keeps all int64 sums exact and in range.
+    // Exact-integer accumulation: contributions are scaled by 2^k and rounded
+    // to integers; every cell sum is bounded by ngrid * maxv, so with
+    // 2^k * ngrid * maxv <= 2^52 all partial sums are exact doubles and the
+    // result does not depend on the order of the LDS atomics.
     const float maxv = __uint_as_float(m.maxv_bits);
     int e = 0;
     frexpf(maxv * (float)max(m.ngrid, 1), &e);
-    const float fscale = ldexpf(1.0f, 61 - e);
-    const float iscale = ldexpf(1.0f, e - 61);
+    const float fscale = ldexpf(1.0f, 52 - e);
+    const double iscale = ldexp(1.0, e - 52);
 
     const int* bst = bstart + (size_t)s * (G + 1);
     const VisRec* vs = sorted + (size_t)s * stride;
@@ -368,8 +397,8 @@ __device__ __forceinline__ void stage1_column(int s, int c, const VisRec* __rest
     const int rlo = -m.r1max, rhi = min(m.r1max, M - 1);
     for (int lo = rlo; lo <= rhi; lo += CH) {
         for (int i = t; i < CH; i += NT) {
-            are[i] = 0ull;
-            aim[i] = 0ull;
+            are[i] = 0.0;
+            aim[i] = 0.0;
         }
         __syncthreads();
         for (int n = d0 + t; n < d1; n += NT) grid_one(vs[n], cc, false, lo, CH, M, fscale, are, aim);
@@ -378,8 +407,8 @@ __device__ __forceinline__ void stage1_column(int s, int c, const VisRec* __rest
         const int nrow = min(CH, rhi - lo + 1);
         for (int i = t; i < nrow; i += NT) {
             const int row = lo + i;
-            const float re = (float)(long long)are[i] * iscale;
-            const float im = (float)(long long)aim[i] * iscale;
+            const float re = (float)(are[i] * iscale);
+            const float im = (float)(aim[i] * iscale);
             fb[lds_pad(row < 0 ? row + G : row)] = make_float2(re, im);
         }
         __syncthreads();
@@ -405,7 +434,7 @@ __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1(const VisRec* __restr
                                                             float2* __restrict__ kt, size_t stride, int I)
 {
     __shared__ float2 fb[lds_len(G)];
-    __shared__ unsigned long long are[CH], aim[CH];
+    __shared__ double are[CH], aim[CH];
     const int s = blockIdx.y;
     const int c = blockIdx.x;
     const SnapMeta m = meta[s];
@@ -422,7 +451,7 @@ __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
     int I, int S, unsigned* work)
 {
     __shared__ float2 fb[lds_len(G)];
-    __shared__ unsigned long long are[CH], aim[CH];
+    __shared__ double are[CH], aim[CH];
     __shared__ unsigned item;
     constexpr unsigned per = G / 2 + 1;
     const unsigned total = (unsigned)S * per;
@@ -521,19 +550,48 @@ __device__ __forceinline__ void sin_shift(const SnapGeom& g, double q1, double q
     const float mm = (float)(g.delta * (q2 - 1.0));
     const float r2 = l * l + mm * mm;
     if (r2 <= 1.0f) {
-        const float np = r2 / (1.0f + sqrtf(1.0f - r2));   // 1 - sqrt(1 - r^2)
+        // 1 - sqrt(1 - r^2) = r^2 / (1 + sqrt(1 - r^2)); hardware sqrt/rcp (1 ulp)
+        const float np = r2 * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - r2));
         o1 = q1 + g.sx * (double)np;
         o2 = q2 + g.sy * (double)np;
     }
 }
 
+// Forward map of source pixel (i, j): affine part in FP64 (exact for the
+// integer offsets), SIN shift from the closed form (FIPkernels.cu:289-376).
+__device__ __forceinline__ void source_position(const SnapGeom& g, int i, int j, int h, double& LL, double& MM)
+{
+    const double a = (double)(i - h), b = (double)(j - h);
+    sin_shift(g, g.m00 * a + g.m01 * b, g.m10 * a + g.m11 * b, LL, MM);
+}
+
+// Exact-integer accumulation: contributions are rounded to integers (scaled
+// by a power of two) and summed as doubles.  Every partial sum stays below
+// 2^53, so each sum is exact and independent of the order of the LDS atomics.
+__device__ __forceinline__ void acc_exact(double* a, double x)
+{
+    atomicAdd(a, rint(x));
+}
+
+// Per target tile: bounding box and per-row ranges of the source pixels whose
+// bilinear footprint can reach the tile, then the forward splat
+// (finalinterp, FIPkernels.cu:378-413) accumulated in an LDS tile with a
+// one-pixel apron.  The (row, column) pairs of all source rows are flattened
+// so every lane of every wave has a source; each wave tracks its row index
+// incrementally and prefetches its next source value.
 template <int TS>
 __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, SnapMeta* __restrict__ meta,
                                                const SnapGeom* __restrict__ geom, float* __restrict__ img,
                                                int I)
 {
     constexpr int NT = 256;
-    __shared__ unsigned long long acc[TS * TS];
+    constexpr int NW = NT / 64;           // waves
+    constexpr int ROW_CAP = 2 * TS + 32;   // source rows handled per pass
+    constexpr int A = TS + 2;             // apron: targets -1 .. TS
+    constexpr int UR = 4;                 // rows in flight per wave
+    __shared__ double acc[A * A];
+    __shared__ int rlo[ROW_CAP], rcnt[ROW_CAP];
+    __shared__ double qb[4];
     __shared__ int bb[4];
     __shared__ unsigned shm[NT / 64];
     const int s = blockIdx.y;
@@ -542,92 +600,173 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     const int R0 = (tile / ntx) * TS, C0 = (tile % ntx) * TS;
     const SnapGeom g = geom[s];
     const int h = I / 2;
-    for (int i = threadIdx.x; i < TS * TS; i += NT) acc[i] = 0ull;
-    if (threadIdx.x == 0) {
-        // Source bounding box: preimages of the corners of the open target
-        // box (C0-1, C0+TS) x (R0-1, R0+TS), padded by the curvature bound.
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) {
         int i0 = 0, i1 = I - 1, j0 = 0, j1 = I - 1;
+        // q-space (affine position - h) box that can reach the open target box
+        // (C0-1, C0+TS) x (R0-1, R0+TS): the target box shifted by the SIN
+        // shift at the tile centre, widened by its variation over the tile.
+        double ql1 = -1e300, qh1 = 1e300, ql2 = -1e300, qh2 = 1e300;
         if (g.curv >= 0.0) {
+            const double t1 = (double)C0 + 0.5 * (TS - 1) - h, t2 = (double)R0 + 0.5 * (TS - 1) - h;
+            double q1 = t1, q2 = t2;
+            for (int it = 0; it < 4; ++it) {
+                double o1, o2;
+                sin_shift(g, q1, q2, o1, o2);
+                q1 = t1 - (o1 - q1);
+                q2 = t2 - (o2 - q2);
+            }
+            double o1, o2;
+            sin_shift(g, q1, q2, o1, o2);
+            const double d1 = o1 - q1, d2 = o2 - q2;
+            // Any q with q + d(q) in the box lies within (halfdiag + residual)/(1 - kappa)
+            // of (q1, q2), so |d(q) - d| <= kappa times that.
+            const double res = fabs(o1 - t1) + fabs(o2 - t2);
+            const double halfdiag = 0.7071067811865476 * (TS + 1);
+            const double dv = g.kappa * (halfdiag + res) / (1.0 - g.kappa) + 0.02;
+            ql1 = (double)(C0 - 1) - h - d1 - dv;
+            qh1 = (double)(C0 + TS) - h - d1 + dv;
+            ql2 = (double)(R0 - 1) - h - d2 - dv;
+            qh2 = (double)(R0 + TS) - h - d2 + dv;
             double mini = 1e300, maxi = -1e300, minj = 1e300, maxj = -1e300;
             for (int k = 0; k < 4; ++k) {
-                const double t1 = ((k & 1) ? (double)(C0 + TS) : (double)(C0 - 1)) - h;
-                const double t2 = ((k & 2) ? (double)(R0 + TS) : (double)(R0 - 1)) - h;
-                double q1 = t1, q2 = t2;
-                for (int it = 0; it < 8; ++it) {
-                    double o1, o2;
-                    sin_shift(g, q1, q2, o1, o2);
-                    q1 = t1 - (o1 - q1);
-                    q2 = t2 - (o2 - q2);
-                }
-                const double a = g.i00 * q1 + g.i01 * q2 + h;
-                const double b = g.i10 * q1 + g.i11 * q2 + h;
+                const double c1 = (k & 1) ? qh1 : ql1, c2 = (k & 2) ? qh2 : ql2;
+                const double a = g.i00 * c1 + g.i01 * c2 + h, b = g.i10 * c1 + g.i11 * c2 + h;
                 mini = fmin(mini, a);
                 maxi = fmax(maxi, a);
                 minj = fmin(minj, b);
                 maxj = fmax(maxj, b);
             }
-            const double ext = (double)(TS + 2);
-            const int margin = 2 + (int)ceil(g.curv * ext * ext);
-            i0 = max(0, (int)floor(mini) - margin);
-            i1 = min(I - 1, (int)ceil(maxi) + margin);
-            j0 = max(0, (int)floor(minj) - margin);
-            j1 = min(I - 1, (int)ceil(maxj) + margin);
+            // clamp before converting (degenerate V can give huge values)
+            mini = fmin(fmax(mini, -4.0), (double)I + 4.0);
+            maxi = fmin(fmax(maxi, -4.0), (double)I + 4.0);
+            minj = fmin(fmax(minj, -4.0), (double)I + 4.0);
+            maxj = fmin(fmax(maxj, -4.0), (double)I + 4.0);
+            i0 = max(0, (int)floor(mini) - 1);
+            i1 = min(I - 1, (int)ceil(maxi) + 1);
+            j0 = max(0, (int)floor(minj) - 1);
+            j1 = min(I - 1, (int)ceil(maxj) + 1);
         }
+        qb[0] = ql1;
+        qb[1] = qh1;
+        qb[2] = ql2;
+        qb[3] = qh2;
         bb[0] = i0;
         bb[1] = i1;
         bb[2] = j0;
         bb[3] = j1;
     }
+    for (int q = tid; q < A * A; q += NT) acc[q] = 0.0;
     __syncthreads();
     const int i0 = bb[0], i1 = bb[1], j0 = bb[2], j1 = bb[3];
+    const int H = j1 - j0 + 1;
     const SnapMeta m = meta[s];
-    const float invn = 1.0f / (float)m.norm_n;   // FIPkernels.cu:391
+    // 2^k * max contribution <= 2^44: sums of up to 256 contributions stay
+    // exact integers below 2^52.
     int e = 0;
-    frexpf(__uint_as_float(m.maxp_bits) * invn * 16.0f, &e);
-    const float fscale = ldexpf(1.0f, 62 - e);
-    const float iscale = ldexpf(1.0f, e - 62);
+    frexpf(__uint_as_float(m.maxp_bits) / (float)m.norm_n, &e);
+    const double vscale = ldexp(1.0, 44 - e) / (double)m.norm_n;   // 1/N of FIPkernels.cu:391
+    const double iscale = ldexp(1.0, e - 44);
     const double lim = (double)(h - 1);
-    if (i1 >= i0 && j1 >= j0) {
-        const int wi = i1 - i0 + 1;
-        const int nsrc = wi * (j1 - j0 + 1);
-        const float* ps = pre + (size_t)s * I * I;
-        for (int idx = threadIdx.x; idx < nsrc; idx += NT) {
-            const int j = j0 + idx / wi;
-            const int i = i0 + idx % wi;
-            const float pv = ps[(size_t)j * I + i];
-            if (pv == 0.0f) continue;
-            const float val = pv * invn * fscale;
-            const double a = (double)(i - h), b = (double)(j - h);
-            double LL, MM;
-            sin_shift(g, g.m00 * a + g.m01 * b, g.m10 * a + g.m11 * b, LL, MM);
-            if (fabs(LL) < lim && fabs(MM) < lim) {
-                const double fl = floor(LL), fm = floor(MM);
-                const float wl1 = (float)(LL - fl), wm1 = (float)(MM - fm);
-                const float wl0 = 1.0f - wl1, wm0 = 1.0f - wm1;
-                const int r0 = (int)fm + h - R0, c0 = (int)fl + h - C0;
-                const int r1 = r0 + (wm1 > 0.0f ? 1 : 0), c1 = c0 + (wl1 > 0.0f ? 1 : 0);
-                // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL)
-                if ((unsigned)r0 < (unsigned)TS && (unsigned)c0 < (unsigned)TS)
-                    atomicAdd(&acc[r0 * TS + c0], (unsigned long long)to_fixed(wl0 * wm0 * val));
-                if ((unsigned)r1 < (unsigned)TS && (unsigned)c0 < (unsigned)TS)
-                    atomicAdd(&acc[r1 * TS + c0], (unsigned long long)to_fixed(wl0 * wm1 * val));
-                if ((unsigned)r0 < (unsigned)TS && (unsigned)c1 < (unsigned)TS)
-                    atomicAdd(&acc[r0 * TS + c1], (unsigned long long)to_fixed(wl1 * wm0 * val));
-                if ((unsigned)r1 < (unsigned)TS && (unsigned)c1 < (unsigned)TS)
-                    atomicAdd(&acc[r1 * TS + c1], (unsigned long long)to_fixed(wl1 * wm1 * val));
+    const float* ps = pre + (size_t)s * I * I;
+    const double ql1 = qb[0], qh1 = qb[1], ql2 = qb[2], qh2 = qb[3];
+    for (int rb = 0; rb < H; rb += ROW_CAP) {
+        const int nrow = min(ROW_CAP, H - rb);
+        // Per source row: the i range whose affine position lies in the q box.
+        for (int r = tid; r < nrow; r += NT) {
+            const double b = (double)(j0 + rb + r - h);
+            double alo = -1e300, ahi = 1e300;
+            const double mA[2] = {g.m00, g.m10}, mB[2] = {g.m01, g.m11};
+            const double lo[2] = {ql1, ql2}, hi[2] = {qh1, qh2};
+            for (int k = 0; k < 2; ++k) {
+                const double cst = mB[k] * b;
+                if (fabs(mA[k]) > 1e-12) {
+                    double x = (lo[k] - cst) / mA[k], y = (hi[k] - cst) / mA[k];
+                    if (x > y) {
+                        const double t = x;
+                        x = y;
+                        y = t;
+                    }
+                    alo = fmax(alo, x);
+                    ahi = fmin(ahi, y);
+                } else if (cst < lo[k] || cst > hi[k]) {
+                    alo = 1e300;
+                    ahi = -1e300;
+                }
+            }
+            alo = fmax(alo, (double)(i0 - h - 2));
+            ahi = fmin(ahi, (double)(i1 - h + 2));
+            int ia = i0, n = 0;
+            if (alo <= ahi) {
+                ia = max(i0, (int)floor(alo + h) - 1);
+                n = max(0, min(i1, (int)ceil(ahi + h) + 1) - ia + 1);
+            }
+            rlo[r] = ia;
+            rcnt[r] = n;
+        }
+        __syncthreads();
+        // Each wave walks UR source rows at a time, each lane two columns per
+        // row per pass: the loads and position chains of up to 2*UR sources
+        // per lane are independent, which hides HBM and FP64 latency.
+        int maxlen = 0;
+        for (int r = 0; r < nrow; ++r) maxlen = max(maxlen, rcnt[r]);
+        for (int r = wid; r < nrow; r += UR * NW) {
+            for (int c0 = 0; c0 < maxlen; c0 += 128) {
+                float pv[UR][2];
+                int ii[UR][2];
+#pragma unroll
+                for (int u = 0; u < UR; ++u) {
+                    const int rr = r + u * NW;
+                    const int ia = rr < nrow ? rlo[rr] : 0, n = rr < nrow ? rcnt[rr] : 0;
+                    const int j = j0 + rb + rr;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const int c = c0 + lane + 64 * k;
+                        ii[u][k] = ia + c;
+                        pv[u][k] = c < n ? ps[(size_t)j * I + ia + c] : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UR; ++u) {
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const float v = pv[u][k];
+                        if (v == 0.0f) continue;
+                        const int i = ii[u][k], j = j0 + rb + r + u * NW;
+                        double LL, MM;
+                        source_position(g, i, j, h, LL, MM);
+                        if (fabs(LL) < lim && fabs(MM) < lim) {
+                            const double fl = floor(LL), fm = floor(MM);
+                            const int ra = (int)fm + h - R0, ca = (int)fl + h - C0;
+                            if ((unsigned)(ra + 1) < (unsigned)(TS + 1) && (unsigned)(ca + 1) < (unsigned)(TS + 1)) {
+                                const double wl1 = LL - fl, wm1 = MM - fm;
+                                const double vs = (double)v * vscale;
+                                const double t0 = (1.0 - wl1) * vs, t1 = wl1 * vs;
+                                // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL); a
+                                // zero weight on an integer position adds an exact zero.
+                                double* p0 = &acc[(ra + 1) * A + (ca + 1)];
+                                acc_exact(p0, t0 * (1.0 - wm1));
+                                acc_exact(p0 + A, t0 * wm1);
+                                acc_exact(p0 + 1, t1 * (1.0 - wm1));
+                                acc_exact(p0 + A + 1, t1 * wm1);
+                            }
+                        }
+                    }
+                }
             }
         }
+        __syncthreads();   // rlo/rcnt are rewritten by the next row batch
     }
-    __syncthreads();
     float* out = img + (size_t)s * I * I;
     float lmax = 0.0f;
-    for (int q = threadIdx.x; q < TS * TS; q += NT) {
-        const float v = (float)(long long)acc[q] * iscale;
-        out[(size_t)(R0 + q / TS) * I + C0 + q % TS] = v;
+    for (int q = tid; q < TS * TS; q += NT) {
+        const int rr = q / TS, cc = q % TS;
+        const float v = (float)(acc[(rr + 1) * A + cc + 1] * iscale);
+        out[(size_t)(R0 + rr) * I + C0 + cc] = v;
         lmax = fmaxf(lmax, v);
     }
-    const unsigned bm = block_max_u<NT>(__float_as_uint(lmax), shm);
-    if (threadIdx.x == 0) atomicMax(&meta[s].maxd_bits, bm);
+    const unsigned bmx = block_max_u<NT>(__float_as_uint(lmax), shm);
+    if (tid == 0) atomicMax(&meta[s].maxd_bits, bmx);
 }
 
 // --------------------------------------------------------------- trigger
@@ -705,17 +844,18 @@ hipError_t launch_bin(const InputView& in, const WorkView& w, const PlanTables& 
                       double scale, hipStream_t st)
 {
     if (nmax <= 0) return hipSuccess;
-    dim3 grid((nmax + 255) / 256, S);
+    dim3 grid((nmax + 256 * kBinVpt - 1) / (256 * kBinVpt), S);
+    const size_t lds = (size_t)p.G * sizeof(int);
     switch (in.vis_type) {
     case kVisF32:
-        hipLaunchKernelGGL(k_bin<float>, grid, dim3(256), 0, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G, scale);
+        hipLaunchKernelGGL(k_bin<float>, grid, dim3(256), lds, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G, scale);
         break;
     case kVisBF16:
-        hipLaunchKernelGGL(k_bin<__hip_bfloat16>, grid, dim3(256), 0, st, in, w.rec, w.bkslot, w.hist, w.meta,
+        hipLaunchKernelGGL(k_bin<__hip_bfloat16>, grid, dim3(256), lds, st, in, w.rec, w.bkslot, w.hist, w.meta,
                            p.G, scale);
         break;
     case kVisF16:
-        hipLaunchKernelGGL(k_bin<__half>, grid, dim3(256), 0, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G,
+        hipLaunchKernelGGL(k_bin<__half>, grid, dim3(256), lds, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G,
                            scale);
         break;
     default:

@@ -11,7 +11,6 @@ int fip_server_drain(fip_server*) { return FIP_ERR_ARG; }
 int fip_server_stats(fip_server*, double*, double*, double*, long*, long*) { return FIP_ERR_ARG; }
 int fip_server_latest(fip_server*, int, float*, long*) { return FIP_ERR_ARG; }
 void fip_server_destroy(fip_server*) {}
-int fip_synth(int, double, size_t, double, double, double, double, size_t, float, int, double, double, double, double, int, const int*, double, double, unsigned long long, float*, float*, float*, float*, int*) { return FIP_ERR_ARG; }
 int fip_fits_read(const char*, float**, long*) { return FIP_ERR_IO; }
 int fip_fits_read_ext(const char*, const char*, float**, long*) { return FIP_ERR_IO; }
 int fip_fits_write(const char*, const float*, long, long) { return FIP_ERR_IO; }
