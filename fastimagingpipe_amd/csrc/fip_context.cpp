@@ -199,8 +199,8 @@ Tuning Context::tuning(size_t image) const
     auto it = tunings_.find(image);
     if (it != tunings_.end()) return it->second;
     Tuning t;
-    t.stage1_chunk = 1 << 30;   // the large LDS band: one gridding pass
-    t.stage2_rows = 4;
+    t.stage1_chunk = 1;         // the small LDS band: two workgroups per CU
+    t.stage2_rows = 2;
     t.splat_tile = 64;
     t.persistent = 0;
     return t;

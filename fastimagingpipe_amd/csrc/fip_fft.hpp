@@ -133,6 +133,18 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ buf, const float2
     constexpr int BPT = (NB + NT - 1) / NT;      // butterflies per thread
     constexpr bool exact = (NB % NT) == 0;
     float2 v[BPT][R];
+    // Twiddles do not depend on the data: issue their (L1/L2-resident) loads
+    // before the LDS loads and the barrier so their latency is hidden.
+    float2 w[BPT][R > 1 ? R - 1 : 1];
+    if (Ns > 1) {
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            const int j = t + b * NT;
+            const int k = (exact || j < NB) ? j % Ns : 0;
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[b][r - 1] = tw[(r - 1) * Ns + k];
+        }
+    }
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
         const int j = t + b * NT;
@@ -149,7 +161,7 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ buf, const float2
             const int k = j % Ns;
             if (Ns > 1) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[(r - 1) * Ns + k]);
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[b][r - 1]);
             }
             Bfly<R>::run(v[b]);
             const int base = (j / Ns) * Ns * R + k;
