@@ -14,9 +14,9 @@ import numpy as np
 
 __all__ = [
     "FipError", "lib", "library_path", "grid_size", "fipipe", "fipipe_lowp", "image",
-    "trigger", "pipe_device", "batch_device", "project", "collapse", "synth",
+    "trigger", "pipe_device", "batch_device", "batch_host", "project", "collapse", "synth",
     "set_profiling", "stage_times", "reset_stage_times", "autotune", "set_tuning",
-    "debug_fft", "fits_read", "fits_write", "Server", "STATUS",
+    "debug_fft", "fits_read", "fits_write", "fits_write3", "Server", "STATUS",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -88,6 +88,8 @@ _sig("fip_fits_read", [ctypes.c_char_p, ctypes.POINTER(_fp), ctypes.POINTER(ctyp
 _sig("fip_fits_read_ext", [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_fp),
                            ctypes.POINTER(ctypes.c_long)])
 _sig("fip_fits_write", [ctypes.c_char_p, _vp, ctypes.c_long, ctypes.c_long])
+_sig("fip_fits_write3", [ctypes.c_char_p, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long])
+_sig("fip_batch_host", [_vp, _vp, _i, _vp, _vp, _vp, _sz, _sz, _sz, _sz, _f, _sz, _vp])
 _sig("fip_free", [_vp], None)
 
 
@@ -199,6 +201,25 @@ def batch_device(vr_ptr: int, vi_ptr: int, vis_type: int, b_ptr: int, v_ptr: int
                                 norm_n, image_size, cell_size, unit_size, out_ptr, stream or None))
 
 
+def batch_host(vis_re, vis_im, bin_, vin, nvis=None, norm_n: int = 0, image_size: int = 0,
+               cell_size: float = 0.0, unit_size: int = 0, vis_type: int = VIS_F32) -> np.ndarray:
+    """fip_batch_host: vis [nsnap][stride], bin [nsnap][stride][2], vin [nsnap][3][3] on the
+    host -> tLISI of every window (s, s+1, s+2), shape (nsnap-2, I/T, I/T)."""
+    if vis_type == VIS_F32:
+        vr, vi = _c(vis_re), _c(vis_im)
+    else:
+        vr = np.ascontiguousarray(vis_re).view(np.uint16)
+        vi = np.ascontiguousarray(vis_im).view(np.uint16)
+    nsnap, stride = vr.shape
+    b, v = _c(bin_), _c(vin)
+    nv = _c(np.full(nsnap, stride) if nvis is None else nvis, np.int32)
+    un = image_size // unit_size if unit_size else 0
+    out = np.zeros((max(nsnap - 2, 0), un, un), np.float32)
+    _check(lib.fip_batch_host(_ptr(vr), _ptr(vi), vis_type, _ptr(b), _ptr(v), _ptr(nv), stride,
+                              nsnap, norm_n, image_size, cell_size, unit_size, _ptr(out)))
+    return out
+
+
 def project(uvw, freqs):
     """SVD-plane projection: (bin [nsamp,2] float32, vin [3,3] float32)."""
     u = _c(uvw, np.float64).reshape(-1, 3)
@@ -302,6 +323,13 @@ def fits_write(path: str, data2d) -> None:
     _check(lib.fip_fits_write(path.encode(), _ptr(a), a.shape[1], a.shape[0]))
 
 
+def fits_write3(path: str, data3d) -> None:
+    a = _c(data3d)
+    if a.ndim != 3:
+        raise ValueError("fits_write3 takes a 3-D array")
+    _check(lib.fip_fits_write3(path.encode(), _ptr(a), a.shape[2], a.shape[1], a.shape[0]))
+
+
 class Server:
     """Serving form: device ring buffers per sub-array stream, HBM-budget admission."""
 
@@ -318,8 +346,9 @@ class Server:
                                      ctypes.byref(self._h)))
         self._streams = list(streams)
 
-    def submit(self, sid: int, vis_re, vis_im, bin_, vin, arrival_s: float) -> bool:
-        """Returns False when the snapshot is refused by admission."""
+    def submit(self, sid: int, vis_re, vis_im, bin_, vin, arrival_s: float = -1.0) -> bool:
+        """arrival_s on time.monotonic()'s clock (negative: now).  Returns False when the
+        snapshot is refused (no free input slot)."""
         vt = self._streams[sid].get("vis_type", VIS_F32)
         if vt == VIS_F32:
             vr, vi = _c(vis_re), _c(vis_im)

@@ -447,6 +447,45 @@ int fip_batch_device(const void* vis_re, const void* vis_im, int vis_type, const
     }
 }
 
+int fip_batch_host(const void* vis_re, const void* vis_im, int vis_type, const float* bin, const float* vin,
+                   const int* nvis, size_t stride, size_t nsnap, size_t norm_n, size_t image_size,
+                   float cell_size, size_t unit_size, float* tlisi)
+{
+    try {
+        if (!vis_re || !vis_im || !bin || !vin || !nvis || !tlisi)
+            return set_error(FIP_ERR_ARG, "null pointer argument");
+        if (int rc = check_vis_type(vis_type)) return rc;
+        if (nsnap < 3) return set_error(FIP_ERR_NSNAP, "batching needs at least 3 snapshots");
+        if (int rc = check_shape(std::max<size_t>(stride, 1), image_size, unit_size, cell_size)) return rc;
+        Context* C = nullptr;
+        if (int rc = Context::get(&C)) return rc;
+        std::lock_guard<std::recursive_mutex> lk(C->mu);
+        const size_t esz = vis_elem_size(vis_type);
+        const size_t vb = nsnap * stride * esz, bb = nsnap * stride * 2 * sizeof(float), mb = nsnap * 9 * sizeof(float);
+        const size_t un = image_size / unit_size;
+        const size_t ob = (nsnap - 2) * un * un * sizeof(float);
+        FIP_HIP(C->in_vr.ensure(vb));
+        FIP_HIP(C->in_vi.ensure(vb));
+        FIP_HIP(C->in_b.ensure(bb));
+        FIP_HIP(C->in_v.ensure(mb));
+        FIP_HIP(C->out.ensure(ob));
+        FIP_HIP(hipStreamSynchronize(C->stream));
+        FIP_HIP(hipMemcpy(C->in_vr.p, vis_re, vb, hipMemcpyHostToDevice));
+        FIP_HIP(hipMemcpy(C->in_vi.p, vis_im, vb, hipMemcpyHostToDevice));
+        FIP_HIP(hipMemcpy(C->in_b.p, bin, bb, hipMemcpyHostToDevice));
+        FIP_HIP(hipMemcpy(C->in_v.p, vin, mb, hipMemcpyHostToDevice));
+        if (int rc = fip_batch_device(C->in_vr.p, C->in_vi.p, vis_type, C->in_b.as<float>(), C->in_v.as<float>(), nvis,
+                                      stride, nsnap, norm_n, image_size, cell_size, unit_size, C->out.as<float>(),
+                                      nullptr))
+            return rc;
+        FIP_HIP(hipStreamSynchronize(C->stream));
+        FIP_HIP(hipMemcpy(tlisi, C->out.p, ob, hipMemcpyDeviceToHost));
+        return FIP_OK;
+    } catch (...) {
+        return set_error(FIP_ERR_HIP, "unexpected exception");
+    }
+}
+
 // ------------------------------------------------------------- profiling
 
 int fip_set_profiling(int on)

@@ -77,8 +77,8 @@ int fip_pipe_device(const float *vis_re, const float *vis_im, const float *bin,
 /*
  * Precision form of fip_pipe: visibilities stored as bfloat16 or IEEE half
  * (vis_type FIP_VIS_BF16 / FIP_VIS_F16; 2-byte elements), converted to FP32
- * on load; gridding accumulates exactly in int64 fixed point and the FFTs
- * run in FP32 (FIPkernels.cu:188-245 for the arithmetic it replaces).
+ * on load; gridding accumulates scaled, integer-rounded contributions in
+ * FP64 (exact, so independent of summation order) and the FFTs run in FP32 (FIPkernels.cu:188-245 for the arithmetic it replaces).
  */
 int fip_pipe_lowp(const void *vis_re, const void *vis_im, int vis_type,
                   const float *bin, const float *vin, float *result,
@@ -120,6 +120,18 @@ int fip_batch_device(const void *vis_re, const void *vis_im, int vis_type,
                      size_t stride, size_t nsnap, size_t norm_n,
                      size_t image_size, float cell_size, size_t unit_size,
                      float *d_tlisi, void *stream);
+
+/*
+ * fip_batch_device with host arrays: copies the nsnap snapshots to the
+ * device, images them and returns (nsnap-2)*(image/tile)^2 tLISI values in
+ * host memory (window k compares snapshots k, k+1, k+2).  Used by the CLI
+ * for combined-FITS input with more than three snapshots.
+ */
+int fip_batch_host(const void *vis_re, const void *vis_im, int vis_type,
+                   const float *bin, const float *vin, const int *nvis,
+                   size_t stride, size_t nsnap, size_t norm_n,
+                   size_t image_size, float cell_size, size_t unit_size,
+                   float *tlisi);
 
 /*
  * SVD-plane uv projection (SVD_MFS.py:38-110; SVDPreprocessKernels.cu:41-206,
@@ -177,7 +189,9 @@ typedef struct {
 int fip_server_create(const fip_stream_config *streams, int nstreams, int ring,
                       size_t hbm_budget_bytes, fip_server **out);
 /* Submit one snapshot (host arrays) for stream `sid` with its arrival time
- * (seconds, caller's clock).  Returns FIP_ERR_BUDGET if it is refused. */
+ * in seconds of the monotonic clock (CLOCK_MONOTONIC, std::steady_clock,
+ * Python's time.monotonic()); a negative value stamps the submit time.
+ * Returns FIP_ERR_BUDGET if it is refused (no free input slot). */
 int fip_server_submit(fip_server *srv, int sid, const void *vis_re,
                       const void *vis_im, const float *bin, const float *vin,
                       double arrival_s);
@@ -219,6 +233,9 @@ int fip_fits_read_ext(const char *path, const char *extname, float **data,
 /* Write a 2-D BITPIX -32 primary image (replacing an existing file). */
 int fip_fits_write(const char *path, const float *data, long naxis1,
                    long naxis2);
+/* Write a 3-D BITPIX -32 primary image (e.g. one tLISI matrix per window). */
+int fip_fits_write3(const char *path, const float *data, long naxis1,
+                    long naxis2, long naxis3);
 void fip_free(void *p);
 
 /* ---- internal checks ----------------------------------------------------- */
