@@ -79,6 +79,12 @@ class DeviceArray:
     def addr(self) -> int:
         return int(self.ptr.value or 0)
 
+    def upload(self, host: np.ndarray) -> None:
+        host = np.ascontiguousarray(host)
+        if host.nbytes > self.nbytes:
+            raise ValueError("upload larger than the allocation")
+        _check(_hip.hipMemcpy(self.ptr, host.ctypes.data_as(_vp), host.nbytes, H2D), "hipMemcpy H2D")
+
     def to_host(self, dtype, shape) -> np.ndarray:
         out = np.empty(shape, dtype=dtype)
         _check(_hip.hipMemcpy(out.ctypes.data_as(_vp), self.ptr, out.nbytes, D2H), "hipMemcpy D2H")

@@ -66,3 +66,34 @@ def rel_rms(a, b) -> float:
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-300))
+
+
+# Synthetic workloads from the library's own generator (fip_synth): a
+# 64-antenna array within 5 km imaged at 256^2 (2016 baselines, tile 16) and a
+# 512-antenna array within 40 km imaged at 4096^2 (130816 baselines, tile 32),
+# 140 MHz, dec -30 deg, power-law sky plus a transient in snapshot 2.
+SMALL = dict(nant=64, radius_m=5000.0, image=256, cell=5.8e-5, tile=16, nsrc=20, noise=0.05,
+             seed=20261202)
+LARGE = dict(nant=512, radius_m=40000.0, image=4096, cell=7.2722e-6, tile=32, nsrc=200, noise=0.1,
+             seed=20261203)
+
+
+def synth_workload(F, w, nsnap=3, transients=(2,), flag=0.0, seed=None, image=None):
+    return F.synth(w["nant"], w["radius_m"], nsnap, 1.0, 140e6, -30.0, -26.7,
+                   image or w["image"], w["cell"], w["nsrc"], 0.01, 1.0, -1.8, w["noise"],
+                   transients=list(transients), transient_flux=0.5, flag_frac_max=flag,
+                   seed=w["seed"] if seed is None else seed)
+
+
+def to_bf16_bits(x):
+    """float32 -> bfloat16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bf16_bits(b):
+    return (np.asarray(b, dtype=np.uint32) << 16).view(np.float32)
+
+
+def triggered(tlisi, tau):
+    return set(zip(*np.nonzero(np.asarray(tlisi) < tau)))

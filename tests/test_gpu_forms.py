@@ -1,0 +1,371 @@
+"""GPU parity of the drop-in path at both workload sizes and of the precision,
+batching, preprocessing and serving forms, against the FP64 oracle and
+against each other (all through the C ABI)."""
+import os
+import subprocess
+import time
+
+import numpy as np
+import pytest
+
+from tests.inputs import (LARGE, SMALL, from_bf16_bits, rel_rms, synth_workload, to_bf16_bits,
+                          triggered)
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TAUS = (0.9, 0.99, 0.999)
+
+
+def _pipe_both(fip, oracle, d, w):
+    nbl = d["nbl"]
+    got = fip.fipipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nbl, w["image"], 3, w["cell"],
+                     w["tile"])
+    ref = oracle.pipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nbl, w["image"], 3, w["cell"],
+                      w["tile"])
+    return got, ref
+
+
+def _check_trigger_sets(got, ref, err):
+    for tau in TAUS:
+        margin = np.abs(ref - tau).min()
+        print(f"tau {tau}: {len(triggered(ref, tau))} tiles, margin {margin:.3g}, err {err:.3g}")
+        assert triggered(got, tau) == triggered(ref, tau), tau
+
+
+def test_small_array_pipe_matches_oracle(fip, oracle):
+    """64 antennas, 2016 baselines, 256^2, tile 16."""
+    d = synth_workload(fip, SMALL)
+    got, ref = _pipe_both(fip, oracle, d, SMALL)
+    err = float(np.abs(got - ref).max())
+    print("tLISI max abs err", err, "min", ref.min())
+    assert err < 2e-5
+    _check_trigger_sets(got, ref, err)
+
+
+@pytest.fixture(scope="module")
+def large(fip):
+    return synth_workload(fip, LARGE)
+
+
+def test_large_array_pipe_matches_oracle(fip, oracle, large):
+    """512 antennas, 130816 baselines, 4096^2, tile 32."""
+    t0 = time.time()
+    got, ref = _pipe_both(fip, oracle, large, LARGE)
+    err = float(np.abs(got - ref).max())
+    print(f"tLISI max abs err {err:.3g}, min tLISI {ref.min():.6f} (oracle {time.time() - t0:.1f} s)")
+    assert err < 2e-5
+    _check_trigger_sets(got, ref, err)
+
+
+def test_large_array_image_rms_and_peak(fip, oracle, large):
+    w = LARGE
+    s = 2
+    d = large
+    pre, img, mx = fip.image(d["vis_re"][s:s + 1], d["vis_im"][s:s + 1], d["bin"][s:s + 1],
+                             d["vin"][s:s + 1], d["nbl"], w["image"], 1, w["cell"])
+    opre, oimg, omax = oracle.image(d["vis_re"][s], d["vis_im"][s], d["bin"][s], d["vin"][s],
+                                    w["image"], w["cell"], want_pre=True)
+    e_pre, e_img = rel_rms(pre[0], opre), rel_rms(img[0], oimg)
+    print(f"4096^2 snapshot 2: pre rel-rms {e_pre:.2e}, img rel-rms {e_img:.2e}, "
+          f"max {mx[0]:.7g} / {omax:.7g}")
+    assert e_pre < 1e-5 and e_img < 1e-4
+    assert np.unravel_index(np.argmax(img[0]), img[0].shape) == np.unravel_index(np.argmax(oimg), oimg.shape)
+    assert abs(mx[0] - omax) <= 1e-5 * omax
+
+
+def test_large_array_deterministic(fip, large):
+    w, d = LARGE, large
+    a = fip.fipipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nbl"], w["image"], 3, w["cell"], w["tile"])
+    b = fip.fipipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nbl"], w["image"], 3, w["cell"], w["tile"])
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("nant,radius,image,cell", [(256, 20000.0, 2048, 2 * 7.2722e-6),
+                                                     (1024, 80000.0, 8192, 7.2722e-6 / 2)])
+def test_other_image_sizes_pipe(fip, oracle, nant, radius, image, cell):
+    """The 2048^2 (32640 baselines) and 8192^2 (523776 baselines) sub-arrays of the
+    serving workload: tLISI and one re-projected image against the oracle."""
+    w = dict(LARGE, nant=nant, radius_m=radius, image=image, cell=cell, tile=32)
+    d = synth_workload(fip, w, seed=20261205 + nant)
+    t0 = time.time()
+    got, ref = _pipe_both(fip, oracle, d, w)
+    err = float(np.abs(got - ref).max())
+    print(f"{image}^2 {d['nbl']} bl: tLISI max abs err {err:.3g}, min {ref.min():.6f} "
+          f"({time.time() - t0:.1f} s)")
+    assert err < 2e-5
+    _check_trigger_sets(got, ref, err)
+    s = 2
+    _, img, mx = fip.image(d["vis_re"][s:s + 1], d["vis_im"][s:s + 1], d["bin"][s:s + 1], d["vin"][s:s + 1],
+                           d["nbl"], image, 1, cell)
+    oimg, omax = oracle.image(d["vis_re"][s], d["vis_im"][s], d["bin"][s], d["vin"][s], image, cell)
+    e = rel_rms(img[0], oimg)
+    print(f"  image rel-rms {e:.2e}, max {mx[0]:.7g} / {omax:.7g}")
+    assert e < 1e-4 and abs(mx[0] - omax) <= 1e-5 * omax
+
+
+@pytest.mark.parametrize("vt", ["bf16", "f16"])
+@pytest.mark.parametrize("which", ["small", "large"])
+def test_precision_form(fip, oracle, vt, which, large):
+    """Visibilities stored in 2 bytes: exact against the FP64 oracle of the stored
+    values; the quantisation error against FP64 of the FP32 input is reported and
+    the triggered-tile set must match it."""
+    w = SMALL if which == "small" else LARGE
+    d = synth_workload(fip, SMALL) if which == "small" else large
+    if vt == "bf16":
+        br, bi = to_bf16_bits(d["vis_re"]), to_bf16_bits(d["vis_im"])
+        qr, qi = from_bf16_bits(br), from_bf16_bits(bi)
+        code = fip.VIS_BF16
+    else:
+        qr, qi = d["vis_re"].astype(np.float16), d["vis_im"].astype(np.float16)
+        br, bi = qr.view(np.uint16), qi.view(np.uint16)
+        qr, qi = qr.astype(np.float32), qi.astype(np.float32)
+        code = fip.VIS_F16
+    nbl = d["nbl"]
+    got = fip.fipipe_lowp(br, bi, code, d["bin"], d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
+    ref_q = oracle.pipe(qr, qi, d["bin"], d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
+    ref = oracle.pipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
+    e_q = float(np.abs(got - ref_q).max())
+    e_full = float(np.abs(got - ref).max())
+    print(f"{which} {vt}: tLISI err vs oracle(stored) {e_q:.2e}, vs oracle(fp32 input) {e_full:.2e}")
+    assert e_q < 2e-5
+    if which == "large":
+        _check_trigger_sets(got, ref, e_full)
+        s = 2
+        _, img, _ = fip.image(qr[s:s + 1], qi[s:s + 1], d["bin"][s:s + 1], d["vin"][s:s + 1], nbl,
+                              w["image"], 1, w["cell"])
+        oimg, _ = oracle.image(d["vis_re"][s], d["vis_im"][s], d["bin"][s], d["vin"][s], w["image"],
+                               w["cell"])
+        print(f"  image rel-rms vs FP64 of the fp32 input: {rel_rms(img[0], oimg):.2e}")
+
+
+def _flagged(fip, w, nsnap, seed, image=None):
+    return synth_workload(fip, w, nsnap=nsnap, transients=(1, nsnap - 2), flag=0.3, seed=seed,
+                          image=image)
+
+
+def test_batch_host_matches_per_window_pipe(fip, oracle):
+    w = SMALL
+    S = 7
+    d = _flagged(fip, w, S, 77)
+    nbl = d["nbl"]
+    nv = d["nvis"]
+    assert nv.min() < nbl
+    out = fip.batch_host(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nv, nbl, w["image"],
+                         w["cell"], w["tile"])
+    assert out.shape == (S - 2, 16, 16)
+    for k in range(S - 2):
+        sl = slice(k, k + 3)
+        one = fip.fipipe(d["vis_re"][sl], d["vis_im"][sl], d["bin"][sl], d["vin"][sl], nbl, w["image"],
+                         3, w["cell"], w["tile"])
+        assert np.abs(out[k] - one).max() < 1e-6, k
+    ref = oracle.pipe(d["vis_re"][2:5], d["vis_im"][2:5], d["bin"][2:5], d["vin"][2:5], nbl, w["image"],
+                      3, w["cell"], w["tile"])
+    assert np.abs(out[2] - ref).max() < 2e-5
+
+
+def test_batch_device_graph_replays(fip):
+    """The captured graph replays on a caller stream, with other library work and
+    host synchronisation in between, and every replay matches the host form."""
+    from fastimagingpipe_amd import hip
+    w = SMALL
+    S = 6
+    d = _flagged(fip, w, S, 78)
+    nbl = d["nbl"]
+    un = w["image"] // w["tile"]
+    vr, vi = hip.DeviceArray(d["vis_re"]), hip.DeviceArray(d["vis_im"])
+    b, v = hip.DeviceArray(d["bin"]), hip.DeviceArray(d["vin"])
+    out = hip.DeviceArray(nbytes=(S - 2) * un * un * 4)
+    poison = np.full((S - 2, un, un), -1.0, np.float32)
+    st = hip.Stream()
+    outs = []
+    for rep in range(4):
+        out.upload(poison)
+        fip.batch_device(vr.addr, vi.addr, fip.VIS_F32, b.addr, v.addr, d["nvis"], nbl, S, nbl, w["image"],
+                         w["cell"], w["tile"], out.addr, stream=st.addr)
+        st.synchronize()
+        outs.append(out.to_host(np.float32, (S - 2, un, un)))
+        # eager library work on the context stream between replays
+        fip.fipipe(d["vis_re"][:3], d["vis_im"][:3], d["bin"][:3], d["vin"][:3], nbl, w["image"], 3,
+                   w["cell"], w["tile"])
+    ref = fip.batch_host(d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nvis"], nbl, w["image"],
+                         w["cell"], w["tile"])
+    for o in outs:
+        assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
+    assert np.abs(outs[0] - ref).max() < 1e-6
+
+
+def test_batch_uneven_counts_match_compacted_oracle(fip, oracle):
+    """nvis[s] rows used, splat normalised by the padded count (norm_n)."""
+    w = SMALL
+    d = _flagged(fip, w, 3, 79)
+    nbl, nv = d["nbl"], d["nvis"]
+    out = fip.batch_host(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nv, nbl, w["image"], w["cell"],
+                         w["tile"])
+    ref = oracle.pipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
+    assert np.abs(out[0] - ref).max() < 2e-5
+
+
+def test_projection_matches_fixtures_and_oracle(fip, oracle):
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "pca_fixtures.npz"))
+    for k in range(4):
+        uvw, freqs = fx[f"uvw{k}"], fx[f"freqs{k}"]
+        B, V = fip.project(uvw, freqs)
+        assert np.allclose(V, fx[f"V{k}"], atol=2e-7), k
+        Bf = fx[f"B{k}"]
+        assert np.abs(B - Bf).max() <= 2e-7 * np.abs(Bf).max(), k
+        ob, ov = oracle.project(uvw, freqs)
+        assert np.abs(B - ob).max() <= 2e-7 * np.abs(ob).max(), k
+
+
+def test_projection_large(fip, oracle):
+    rng = np.random.default_rng(4)
+    uvw = rng.normal(size=(130816, 3)) * np.array([2e4, 2e4, 1e3])
+    B, V = fip.project(uvw, [140e6])
+    ob, ov = oracle.project(uvw, [140e6])
+    assert np.abs(V - ov).max() < 2e-7
+    assert np.abs(B - ob).max() <= 2e-7 * np.abs(ob).max()
+
+
+def test_collapse_matches_oracle(fip, oracle):
+    rng = np.random.default_rng(8)
+    n = 100_003
+    xx = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    yy = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    f0 = (rng.uniform(size=n) < 0.2).astype(np.uint8)
+    f3 = (rng.uniform(size=n) < 0.2).astype(np.uint8)
+    w0 = rng.uniform(0, 2, n).astype(np.float32)
+    w3 = rng.uniform(0, 2, n).astype(np.float32)
+    w3[:50] = 0.0
+    w0[:50] = 0.0
+    got = fip.collapse(xx, yy, f0, f3, w0, w3)
+    ref = oracle.collapse(xx, yy, f0, f3, w0, w3)
+    assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
+    assert np.all(got[:50] == 0)
+
+
+def test_degenerate_input_is_reported(fip):
+    w = SMALL
+    nbl = 2016
+    z = np.zeros((3, nbl), np.float32)
+    b = np.zeros((3, nbl, 2), np.float32)
+    v = np.tile(np.eye(3, dtype=np.float32), (3, 1, 1))
+    with pytest.raises(fip.FipError) as ei:
+        fip.fipipe(z, z, b, v, nbl, w["image"], 3, w["cell"], w["tile"])
+    assert ei.value.status == -9
+
+
+def test_serving_streams_match_pipe(fip):
+    streams = [dict(num_baselines=2016, image_size=256, cell_size=SMALL["cell"], unit_size=16),
+               dict(num_baselines=2016, image_size=512, cell_size=SMALL["cell"] / 2, unit_size=32,
+                    vis_type=fip.VIS_BF16)]
+    d0 = synth_workload(fip, SMALL, nsnap=6, transients=(3,), seed=90)
+    d1 = synth_workload(fip, dict(SMALL, image=512, cell=SMALL["cell"] / 2), nsnap=5, transients=(2,), seed=91)
+    srv = fip.Server(streams, ring=4, hbm_budget_bytes=4 << 30)
+    try:
+        for s in range(6):
+            assert srv.submit(0, d0["vis_re"][s], d0["vis_im"][s], d0["bin"][s], d0["vin"][s], time.monotonic())
+            if s < 5:
+                br, bi = to_bf16_bits(d1["vis_re"][s]), to_bf16_bits(d1["vis_im"][s])
+                while not srv.submit(1, br, bi, d1["bin"][s], d1["vin"][s], time.monotonic()):
+                    srv.drain()
+        srv.drain()
+        st = srv.stats()
+        print("serving stats", st)
+        assert st["processed"] == 11
+        assert 0 < st["p50_ms"] <= st["p99_ms"]
+        t0, win0 = srv.latest(0)
+        t1, win1 = srv.latest(1)
+        assert win0 == 3 and win1 == 2
+        e0 = fip.fipipe(d0["vis_re"][3:6], d0["vis_im"][3:6], d0["bin"][3:6], d0["vin"][3:6], 2016, 256, 3,
+                        SMALL["cell"], 16)
+        assert np.abs(t0 - e0).max() < 1e-6
+        e1 = fip.fipipe_lowp(to_bf16_bits(d1["vis_re"][2:5]), to_bf16_bits(d1["vis_im"][2:5]), fip.VIS_BF16,
+                             d1["bin"][2:5], d1["vin"][2:5], 2016, 512, 3, SMALL["cell"] / 2, 32)
+        assert np.abs(t1 - e1).max() < 1e-6
+    finally:
+        srv.close()
+
+
+def test_serving_budget_refuses(fip):
+    with pytest.raises(fip.FipError) as ei:
+        fip.Server([dict(num_baselines=130816, image_size=4096, cell_size=7.27e-6, unit_size=32)],
+                   ring=8, hbm_budget_bytes=64 << 20)
+    assert ei.value.status == -7
+
+
+def _cli(*args):
+    return subprocess.run([os.path.join(ROOT, "sharedlibrary_gpu"), *map(str, args)], capture_output=True,
+                          text=True, timeout=120)
+
+
+def test_cli_end_to_end(fip, tmp_path):
+    w = SMALL
+    d = synth_workload(fip, w)
+    nbl = d["nbl"]
+    p = {k: str(tmp_path / f"{k}.fits") for k in ("vr", "vi", "b", "v", "o1", "o2")}
+    fip.fits_write(p["vr"], d["vis_re"])
+    fip.fits_write(p["vi"], d["vis_im"])
+    fip.fits_write(p["b"], d["bin"].reshape(3, -1))
+    fip.fits_write(p["v"], d["vin"].reshape(3, 9))
+    r = _cli(p["vr"], p["vi"], p["b"], p["v"], w["image"], nbl, w["cell"], 3, w["tile"], p["o1"])
+    assert r.returncode == 0, r.stderr
+    assert "Time elapsed" in r.stdout
+    r = _cli(p["vr"], p["vi"], p["b"], p["v"], w["image"], nbl, 1.4e8, w["cell"], 3, w["tile"], p["o2"])
+    assert r.returncode == 0, r.stderr
+    ref = fip.fipipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
+    for k in ("o1", "o2"):
+        got = fip.fits_read(p[k])
+        assert got.shape == (1, 16, 16)
+        assert np.array_equal(got[0], ref)
+
+
+def _write_combined(path, d, S):
+    """Three IMAGE extensions as the preprocessor writes them, built with numpy."""
+    def card(k, v=None):
+        if v is None:
+            s = k
+        elif isinstance(v, str):
+            s = f"{k:<8}= '{v:<8}'"
+        elif isinstance(v, bool):
+            s = f"{k:<8}= {'T' if v else 'F':>20}"
+        else:
+            s = f"{k:<8}= {v!s:>20}"
+        return s.ljust(80)[:80]
+
+    def pad(b, f):
+        return b + f * ((-len(b)) % 2880)
+
+    def hdu(a, ext):
+        cards = [card("XTENSION", "IMAGE") if ext else card("SIMPLE", True), card("BITPIX", -32),
+                 card("NAXIS", a.ndim)]
+        cards += [card(f"NAXIS{i + 1}", n) for i, n in enumerate(a.shape[::-1])]
+        cards += [card("PCOUNT", 0), card("GCOUNT", 1), card("EXTNAME", ext)] if ext else [card("EXTEND", True)]
+        cards.append(card("END"))
+        return pad("".join(cards).encode(), b" ") + pad(a.astype(">f4").tobytes(), b"\0")
+
+    vis = np.stack([d["vis_re"][:S], d["vis_im"][:S]], axis=-1)      # (S, N, 2)
+    blob = (hdu(np.zeros(1, np.float32), None) + hdu(d["vin"][:S], "TRANSFORMATION MATRICES")
+            + hdu(vis, "VISIBILITIES") + hdu(d["bin"][:S], "R-COORDINATES"))
+    with open(path, "wb") as f:
+        f.write(blob)
+
+
+def test_cli_combined_all_windows(fip, tmp_path):
+    w = SMALL
+    S = 5
+    d = synth_workload(fip, w, nsnap=S, transients=(2,), seed=95)
+    nbl = d["nbl"]
+    src = str(tmp_path / "combined.fits")
+    _write_combined(src, d, S)
+    o1, o2 = str(tmp_path / "one.fits"), str(tmp_path / "all.fits")
+    r = _cli("--combined", src, w["image"], w["cell"], w["tile"], o1)
+    assert r.returncode == 0, r.stderr
+    r = _cli("--combined", src, w["image"], w["cell"], w["tile"], o2, "--all-windows")
+    assert r.returncode == 0, r.stderr
+    one = fip.fits_read(o1)[0]
+    cube = fip.fits_read(o2)
+    assert cube.shape == (S - 2, 16, 16)
+    ref = fip.batch_host(d["vis_re"], d["vis_im"], d["bin"], d["vin"], None, nbl, w["image"], w["cell"],
+                         w["tile"])
+    assert np.array_equal(cube, ref)
+    assert np.abs(one - ref[0]).max() < 1e-6
