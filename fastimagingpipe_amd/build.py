@@ -39,7 +39,6 @@ SOURCES = [
     "fip_synth.cpp",
     "fip_fits.cpp",
     "fip_serve.cpp",
-    "fip_stubs.cpp",
 ]
 
 

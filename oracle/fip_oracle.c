@@ -20,8 +20,8 @@
  *   fo_max, fo_trigger : constant images give tLISI = 1; a single changed
  *       pixel gives the hand-computed tile value; numpy brute force.
  *   fo_project : against scikit-learn PCA fixtures (tests/golden/).
- *   fo_collapse : against a numpy restatement of the masked weighted mean
- *       on random inputs including all-flagged samples.
+ *   fo_collapse : hand-computed cases: equal weights give the mean, one
+ *       side flagged gives the other, all flagged or zero weights give 0.
  */
 #include "fip_oracle.h"
 
