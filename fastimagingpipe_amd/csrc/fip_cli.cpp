@@ -11,6 +11,8 @@
 //       TRANSFORMATION MATRICES (3,3,S), VISIBILITIES (2,N,S), R-COORDINATES (2,N,S).
 //       Default: snapshots 0..2 -> one tLISI matrix; --all-windows: one matrix per
 //       window (s, s+1, s+2) written as a (I/T, I/T, S-2) cube.
+//   Either form may end with  --tau X : list the triggered tiles, tLISI < X
+//   (the reference computes tLISI but defines no threshold).
 #include "fip.h"
 
 #include <cerrno>
@@ -30,7 +32,8 @@ void usage()
                  "                         Number_of_Baselines [Frequency] Cell_Size Number_of_Snapshots\n"
                  "                         Tile_Size Output.fits\n"
                  "       sharedlibrary_gpu --combined input.fits Image_Size Cell_Size Tile_Size Output.fits\n"
-                 "                         [--all-windows]\n");
+                 "                         [--all-windows]\n"
+                 "       either form may end with --tau X (list tiles with tLISI < X)\n");
 }
 
 bool parse_size(const char* s, size_t* out)
@@ -53,6 +56,22 @@ bool parse_float(const char* s, float* out)
     if (errno || *end) return false;
     *out = v;
     return true;
+}
+
+// --tau X, stripped from the argument list by main (negative: not given).
+float g_tau = -1.0f;
+
+void report_triggers(const float* t, size_t un, long window)
+{
+    if (g_tau < 0.0f) return;
+    size_t n = 0;
+    for (size_t q = 0; q < un * un; ++q) n += t[q] < g_tau;
+    if (window >= 0)
+        std::printf("window %ld: %zu tile(s) with tLISI < %g\n", window, n, (double)g_tau);
+    else
+        std::printf("%zu tile(s) with tLISI < %g\n", n, (double)g_tau);
+    for (size_t q = 0; q < un * un; ++q)
+        if (t[q] < g_tau) std::printf("  tile row %zu col %zu tLISI %.9g\n", q / un, q % un, (double)t[q]);
 }
 
 struct Fits {
@@ -118,6 +137,7 @@ int run_separate(int argc, char** argv)
     }
     if (rc == FIP_ERR_DEGENERATE) std::fprintf(stderr, "WARNING: %s\n", fip_last_error());
     std::printf("Time elapsed: %.3f ms\n", std::chrono::duration<double, std::milli>(t1 - t0).count());
+    report_triggers(result.data(), un, -1);
     if (fip_fits_write(out, result.data(), (long)un, (long)un) != FIP_OK) {
         std::fprintf(stderr, "Error writing FITS image: %s\n", fip_last_error());
         return 1;
@@ -170,6 +190,7 @@ int run_combined(int argc, char** argv)
     if (!all) {
         std::vector<float> result(un * un);
         rc = fip_pipe(vr.data(), vi.data(), r.data, v.data, result.data(), N, image, S, cell, tile);
+        if (rc == FIP_OK) report_triggers(result.data(), un, -1);
         if (rc == FIP_OK || rc == FIP_ERR_DEGENERATE)
             rc = fip_fits_write(argv[6], result.data(), (long)un, (long)un) == FIP_OK ? rc : FIP_ERR_IO;
     } else {
@@ -183,6 +204,8 @@ int run_combined(int argc, char** argv)
         std::vector<float> result((S - 2) * un * un);
         rc = fip_batch_host(vr.data(), vi.data(), FIP_VIS_F32, r.data, v.data, nvis.data(), N, S, N, image, cell,
                             tile, result.data());
+        if (rc == FIP_OK)
+            for (size_t w = 0; w + 2 < S; ++w) report_triggers(result.data() + w * un * un, un, (long)w);
         if (rc == FIP_OK) rc = fip_fits_write3(argv[6], result.data(), (long)un, (long)un, (long)(S - 2));
     }
     if (rc != FIP_OK && rc != FIP_ERR_DEGENERATE) {
@@ -196,6 +219,13 @@ int run_combined(int argc, char** argv)
 
 int main(int argc, char** argv)
 {
+    if (argc >= 3 && std::strcmp(argv[argc - 2], "--tau") == 0) {
+        if (!parse_float(argv[argc - 1], &g_tau) || !(g_tau >= 0.0f)) {
+            std::fprintf(stderr, "ERROR: --tau needs a non-negative number\n");
+            return 1;
+        }
+        argc -= 2;
+    }
     if (argc >= 2 && std::strcmp(argv[1], "--combined") == 0) return run_combined(argc, argv);
     if (argc != 11 && argc != 12) {
         usage();

@@ -439,11 +439,23 @@ __device__ __forceinline__ void stage1_column(int s, int c, const VisRec* __rest
     // Crop y in [-h, h) and apply the row correction 1/(c(|y|) norm).
     const int h = I / 2;
     float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
-    for (int yp = t; yp < I; yp += NT) {
-        const int y = yp - h;
-        const float2 v = fb[lds_pad(y < 0 ? y + G : y)];
-        const float cy = corr[y < 0 ? -y : y];
-        ks[((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)] = make_float2(v.x * cy, v.y * cy);
+    for (int y0 = t; y0 < I; y0 += 8 * NT) {
+        float cv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int y = y0 + u * NT - h;
+            cv[u] = (y0 + u * NT < I) ? corr[y < 0 ? -y : y] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int yp = y0 + u * NT;
+            if (yp < I) {
+                const int y = yp - h;
+                const float2 v = fb[lds_pad(y < 0 ? y + G : y)];
+                ks[((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)] =
+                    make_float2(v.x * cv[u], v.y * cv[u]);
+            }
+        }
     }
 }
 
@@ -457,8 +469,11 @@ __global__ __launch_bounds__(FftPlan<G>::nt, 2) void k_stage1(const VisRec* __re
 {
     __shared__ float2 fb[lds_len(G)];
     __shared__ double are[CH], aim[CH];
-    const int s = blockIdx.y;
-    const int c = blockIdx.x;
+    // Column-major over snapshots: the heavily populated low columns of
+    // every snapshot are dispatched first, so none of them ends the launch.
+    const int S = (int)(gridDim.x / (G / 2 + 1));
+    const int s = (int)(blockIdx.x % S);
+    const int c = (int)(blockIdx.x / S);
     const SnapMeta m = meta[s];
     if (c > m.rmax) return;
     stage1_column<G, CH>(s, c, sorted, bstart, m, tw, corr, kt, stride, I, fb, are, aim);
@@ -483,7 +498,7 @@ __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
         const unsigned it = item;
         __syncthreads();
         if (it >= total) break;
-        const int s = (int)(it / per), c = (int)(it % per);
+        const int s = (int)(it % (unsigned)S), c = (int)(it / (unsigned)S);
         const SnapMeta m = meta[s];
         if (c > m.rmax) continue;
         stage1_column<G, CH>(s, c, sorted, bstart, m, tw, corr, kt, stride, I, fb, are, aim);
@@ -514,12 +529,28 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
     float2* z = zb + grp * L;
     const float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
 
-    // Gather K[y][c], c = 0..cmax, for the ROWS rows; zero the rest.
+    // Gather K[y][c], c = 0..cmax, for the ROWS rows; zero the rest.  The
+    // loads of GB consecutive passes are issued before any LDS store, so one
+    // HBM/L2 latency is paid per GB passes instead of per pass.
     for (int c = cmax + 1 + t; c <= M; c += NT) z[lds_pad(c)] = make_float2(0.0f, 0.0f);
-    for (int L2 = threadIdx.x; L2 < (cmax + 1) * ROWS; L2 += NT * ROWS) {
-        const int c = L2 / ROWS, r = L2 % ROWS;
-        const int yp = y0 + r;
-        zb[r * L + lds_pad(c)] = ks[((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)];
+    {
+        constexpr int GB = 8;
+        const int nel = (cmax + 1) * ROWS;
+        const float2* kt0 = ks + (size_t)(y0 / kKTile) * (M + 1) * kKTile + (y0 % kKTile);
+        for (int base = threadIdx.x; base < nel; base += NT * ROWS * GB) {
+            float2 v[GB];
+#pragma unroll
+            for (int u = 0; u < GB; ++u) {
+                const int L2 = base + u * NT * ROWS;
+                const int c = L2 / ROWS, r = L2 % ROWS;
+                if (L2 < nel) v[u] = kt0[(size_t)c * kKTile + r];
+            }
+#pragma unroll
+            for (int u = 0; u < GB; ++u) {
+                const int L2 = base + u * NT * ROWS;
+                if (L2 < nel) zb[(L2 % ROWS) * L + lds_pad(L2 / ROWS)] = v[u];
+            }
+        }
     }
     __syncthreads();
     // C2R split: Z[k] = E[k] + i O[k], E = K[k] + K~[k+M], O = (K[k] - K~[k+M]) e^{2 pi i k/G},
@@ -547,14 +578,27 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
     const int h = I / 2;
     float* out = pre + (size_t)s * I * I + (size_t)(y0 + grp) * I;
     float lmax = 0.0f;
-    for (int col = t; col < I; col += NT) {
-        const int x = col - h;
-        const int mm = x < 0 ? x + G : x;
-        const float2 v = z[lds_pad(mm >> 1)];
-        const float d = (mm & 1) ? v.y : v.x;
-        const float val = fabsf(d) * corr[x < 0 ? -x : x];
-        out[col] = val;
-        lmax = fmaxf(lmax, val);
+    // Eight columns per thread per pass: the correction loads of a pass are
+    // issued together.
+    for (int c0 = t; c0 < I; c0 += 8 * NT) {
+        float cv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int x = c0 + u * NT - h;
+            cv[u] = (c0 + u * NT < I) ? corr[x < 0 ? -x : x] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int col = c0 + u * NT;
+            if (col < I) {
+                const int x = col - h;
+                const int mm = x < 0 ? x + G : x;
+                const float2 v = z[lds_pad(mm >> 1)];
+                const float val = fabsf((mm & 1) ? v.y : v.x) * cv[u];
+                out[col] = val;
+                lmax = fmaxf(lmax, val);
+            }
+        }
     }
     const unsigned bm = block_max_u<NT * ROWS>(__float_as_uint(lmax), shm);
     if (threadIdx.x == 0) atomicMax(&meta[s].maxp_bits, bm);
@@ -598,9 +642,12 @@ __device__ __forceinline__ void acc_exact(double* a, double x)
 // Per target tile: bounding box and per-row ranges of the source pixels whose
 // bilinear footprint can reach the tile, then the forward splat
 // (finalinterp, FIPkernels.cu:378-413) accumulated in an LDS tile with a
-// one-pixel apron.  The (row, column) pairs of all source rows are flattened
-// so every lane of every wave has a source; each wave tracks its row index
-// incrementally and prefetches its next source value.
+// one-pixel apron.  The candidate sources of a batch of rows are numbered
+// row by row; lane work is a contiguous range of that numbering, so every
+// lane of every wave has a source whatever the rotation of the map.  A
+// lane finds its row from a table of the rows in which each run of 64
+// numbers starts (usually no further step) and keeps eight independent
+// loads and position chains in flight.
 template <int TS>
 __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, SnapMeta* __restrict__ meta,
                                                const SnapGeom* __restrict__ geom, float* __restrict__ img,
@@ -610,9 +657,11 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     constexpr int NW = NT / 64;           // waves
     constexpr int ROW_CAP = 2 * TS + 32;   // source rows handled per pass
     constexpr int A = TS + 2;             // apron: targets -1 .. TS
-    constexpr int UR = 4;                 // rows in flight per wave
+    constexpr int UB = 8;                 // sources in flight per lane
+    constexpr int CHK = 512;              // runs of 64 numbered sources per window
     __shared__ double acc[A * A];
-    __shared__ int rlo[ROW_CAP], rcnt[ROW_CAP];
+    __shared__ int rlo[ROW_CAP], rcnt[ROW_CAP], rofs[ROW_CAP + 1];
+    __shared__ unsigned char crow[CHK];   // row in which each run of 64 starts
     __shared__ double qb[4];
     __shared__ int bb[4];
     __shared__ unsigned shm[NT / 64];
@@ -727,55 +776,76 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
             rcnt[r] = n;
         }
         __syncthreads();
-        // Each wave walks UR source rows at a time, each lane two columns per
-        // row per pass: the loads and position chains of up to 2*UR sources
-        // per lane are independent, which hides HBM and FP64 latency.
-        int maxlen = 0;
-        for (int r = 0; r < nrow; ++r) maxlen = max(maxlen, rcnt[r]);
-        for (int r = wid; r < nrow; r += UR * NW) {
-            for (int c0 = 0; c0 < maxlen; c0 += 128) {
-                float pv[UR][2];
-                int ii[UR][2];
+        // Exclusive prefix of the per-row counts (wave 0).
+        if (wid == 0) {
+            int run = 0;
+            for (int b0 = 0; b0 < nrow; b0 += 64) {
+                const int r = b0 + lane;
+                const int v = r < nrow ? rcnt[r] : 0;
+                int x = v;
 #pragma unroll
-                for (int u = 0; u < UR; ++u) {
-                    const int rr = r + u * NW;
-                    const int ia = rr < nrow ? rlo[rr] : 0, n = rr < nrow ? rcnt[rr] : 0;
-                    const int j = j0 + rb + rr;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int y = __shfl_up(x, d);
+                    if (lane >= d) x += y;
+                }
+                if (r < nrow) rofs[r] = run + x - v;
+                run += __shfl(x, 63);
+            }
+            if (lane == 0) rofs[nrow] = run;
+        }
+        __syncthreads();
+        const int total = rofs[nrow];
+        for (int f0 = 0; f0 < total; f0 += CHK * 64) {
+            const int nf = min(CHK * 64, total - f0);
+            // Row of the first number of every run of 64 in [f0, f0 + nf).
+            for (int r = tid; r < nrow; r += NT) {
+                const int a = max(rofs[r], f0), b = min(rofs[r] + rcnt[r], f0 + nf);
+                for (int k = (a - f0 + 63) >> 6; (k << 6) < b - f0; ++k) crow[k] = (unsigned char)r;
+            }
+            __syncthreads();
+            for (int base = 0; base < nf; base += NT * UB) {
+                int ii[UB], jj[UB];
+                float pv[UB];
 #pragma unroll
-                    for (int k = 0; k < 2; ++k) {
-                        const int c = c0 + lane + 64 * k;
-                        ii[u][k] = ia + c;
-                        pv[u][k] = c < n ? ps[(size_t)j * I + ia + c] : 0.0f;
+                for (int u = 0; u < UB; ++u) {
+                    const int fl = base + u * NT + tid;
+                    ii[u] = -1;
+                    jj[u] = 0;
+                    if (fl < nf) {
+                        const int f = f0 + fl;
+                        int r = crow[fl >> 6];
+                        while (rofs[r + 1] <= f) ++r;
+                        ii[u] = rlo[r] + (f - rofs[r]);
+                        jj[u] = j0 + rb + r;
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < UR; ++u) {
+                for (int u = 0; u < UB; ++u) pv[u] = ii[u] >= 0 ? ps[(size_t)jj[u] * I + ii[u]] : 0.0f;
 #pragma unroll
-                    for (int k = 0; k < 2; ++k) {
-                        const float v = pv[u][k];
-                        if (v == 0.0f) continue;
-                        const int i = ii[u][k], j = j0 + rb + r + u * NW;
-                        double LL, MM;
-                        source_position(g, i, j, h, LL, MM);
-                        if (fabs(LL) < lim && fabs(MM) < lim) {
-                            const double fl = floor(LL), fm = floor(MM);
-                            const int ra = (int)fm + h - R0, ca = (int)fl + h - C0;
-                            if ((unsigned)(ra + 1) < (unsigned)(TS + 1) && (unsigned)(ca + 1) < (unsigned)(TS + 1)) {
-                                const double wl1 = LL - fl, wm1 = MM - fm;
-                                const double vs = (double)v * vscale;
-                                const double t0 = (1.0 - wl1) * vs, t1 = wl1 * vs;
-                                // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL); a
-                                // zero weight on an integer position adds an exact zero.
-                                double* p0 = &acc[(ra + 1) * A + (ca + 1)];
-                                acc_exact(p0, t0 * (1.0 - wm1));
-                                acc_exact(p0 + A, t0 * wm1);
-                                acc_exact(p0 + 1, t1 * (1.0 - wm1));
-                                acc_exact(p0 + A + 1, t1 * wm1);
-                            }
+                for (int u = 0; u < UB; ++u) {
+                    const float v = pv[u];
+                    if (v == 0.0f) continue;
+                    double LL, MM;
+                    source_position(g, ii[u], jj[u], h, LL, MM);
+                    if (fabs(LL) < lim && fabs(MM) < lim) {
+                        const double fl = floor(LL), fm = floor(MM);
+                        const int ra = (int)fm + h - R0, ca = (int)fl + h - C0;
+                        if ((unsigned)(ra + 1) < (unsigned)(TS + 1) && (unsigned)(ca + 1) < (unsigned)(TS + 1)) {
+                            const double wl1 = LL - fl, wm1 = MM - fm;
+                            const double vs = (double)v * vscale;
+                            const double t0 = (1.0 - wl1) * vs, t1 = wl1 * vs;
+                            // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL); a
+                            // zero weight on an integer position adds an exact zero.
+                            double* p0 = &acc[(ra + 1) * A + (ca + 1)];
+                            acc_exact(p0, t0 * (1.0 - wm1));
+                            acc_exact(p0 + A, t0 * wm1);
+                            acc_exact(p0 + 1, t1 * (1.0 - wm1));
+                            acc_exact(p0 + A + 1, t1 * wm1);
                         }
                     }
                 }
             }
+            __syncthreads();   // crow is rewritten by the next window
         }
         __syncthreads();   // rlo/rcnt are rewritten by the next row batch
     }
@@ -809,15 +879,32 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
     const float* d3 = img + (size_t)s3 * I * I;
     const float C = 1e-6f;   // FIPkernels.cu:811
     float sum = 0.0f, mx = 0.0f, sr = 0.0f;
-    for (int p = threadIdx.x; p < T * T; p += NT) {
-        const size_t q = (size_t)(ti * T + p / T) * I + (size_t)(tj * T + p % T);
-        const float a = d1[q], b = d2[q], c = d3[q];
+    auto px = [&](float a, float b, float c) {
         const float dd = fabsf(fabsf(a - b) - fabsf(b - c));
         const float b2 = (b <= 0.0f) ? C : b;
         sum += dd;
         mx = fmaxf(mx, dd);
         const float ratio = dd / b2;
         sr += (ratio < 1.0f) ? ratio : 1.0f;
+    };
+    if ((T & 3) == 0) {
+        // 16-byte loads: rows of a tile start at multiples of 4 floats.
+        const int T4 = T >> 2;
+        for (int p = threadIdx.x; p < T * T4; p += NT) {
+            const size_t q = (size_t)(ti * T + p / T4) * I + (size_t)(tj * T + 4 * (p % T4));
+            const float4 a = *reinterpret_cast<const float4*>(d1 + q);
+            const float4 b = *reinterpret_cast<const float4*>(d2 + q);
+            const float4 c = *reinterpret_cast<const float4*>(d3 + q);
+            px(a.x, b.x, c.x);
+            px(a.y, b.y, c.y);
+            px(a.z, b.z, c.z);
+            px(a.w, b.w, c.w);
+        }
+    } else {
+        for (int p = threadIdx.x; p < T * T; p += NT) {
+            const size_t q = (size_t)(ti * T + p / T) * I + (size_t)(tj * T + p % T);
+            px(d1[q], d2[q], d3[q]);
+        }
     }
     sum = wave_sum_f(sum);
     mx = wave_max_f(mx);
@@ -913,7 +1000,7 @@ static void stage1_dispatch(const WorkView& w, const PlanTables& p, int S, bool 
         hipLaunchKernelGGL((k_stage1_persistent<G, CH>), dim3(nb), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
                            p.tw_g, p.corr, w.kt, w.stride, p.I, S, w.work);
     } else {
-        hipLaunchKernelGGL((k_stage1<G, CH>), dim3(G / 2 + 1, S), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
+        hipLaunchKernelGGL((k_stage1<G, CH>), dim3((G / 2 + 1) * S), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
                            p.tw_g, p.corr, w.kt, w.stride, p.I);
     }
 }

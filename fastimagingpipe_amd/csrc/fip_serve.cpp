@@ -5,8 +5,9 @@
 // A submitted snapshot is copied to a free input slot, imaged into the next
 // ring slot and, from the third snapshot on, triggered against the two
 // previous ring slots (the window of FIPkernels.cu:806-851); the tLISI is
-// copied to pinned memory and a host function enqueued behind it stamps the
-// completion time.  Admission: device memory for all streams must fit the HBM
+// copied to pinned memory and an event recorded behind it; a completion
+// thread polls the events of the snapshots in flight and stamps each
+// completion time, so no host callback ever stalls a stream.  Admission: device memory for all streams must fit the HBM
 // budget at creation; a snapshot is refused when all of its stream's input
 // slots are still in flight.
 #include "fip.h"
@@ -18,6 +19,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -45,6 +47,7 @@ struct InSlot {
     PinnedBuf host;        // staging for the four input arrays
     PinnedBuf result;      // pinned tLISI of this snapshot
     std::atomic<int> busy{0};
+    hipEvent_t done = nullptr;   // recorded after the tLISI copy
     double arrival = 0.0;
     long window = -1;      // window index produced by this snapshot, -1 if none
     int stream_id = 0;
@@ -75,15 +78,18 @@ struct fip_server {
     std::mutex mu;                 // guards latencies / latest / counters
     std::vector<double> lat_ms;
     long processed = 0, refused = 0, inflight = 0;
+    // completion tracking
+    std::mutex pmu;
+    std::condition_variable pcv;
+    std::vector<InSlot*> pending;  // submitted, event not yet seen complete
+    bool stop = false;
+    std::thread poller;
 };
 
 namespace {
 
-void on_complete(void* p)
+void on_complete(InSlot* sl, double t)
 {
-    // Runs on a HIP runtime thread after the tLISI copy: no HIP calls here.
-    InSlot* sl = static_cast<InSlot*>(p);
-    const double t = now_s();
     fip_server* srv = sl->srv;
     {
         std::lock_guard<std::mutex> lk(srv->mu);
@@ -100,6 +106,35 @@ void on_complete(void* p)
         }
     }
     sl->busy.store(0);
+}
+
+// Completion thread: polls the events of the snapshots in flight (any
+// stream, any order) and completes each as soon as its event has fired.
+void poll_loop(fip_server* srv)
+{
+    std::vector<InSlot*> work;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(srv->pmu);
+            srv->pcv.wait(lk, [&] { return srv->stop || !srv->pending.empty(); });
+            if (srv->stop && srv->pending.empty()) return;
+            work = srv->pending;
+        }
+        bool any = false;
+        for (InSlot* sl : work) {
+            const hipError_t q = hipEventQuery(sl->done);
+            if (q == hipErrorNotReady) continue;
+            // hipSuccess, or an error that will not clear: complete the slot either way.
+            const double t = now_s();
+            {
+                std::lock_guard<std::mutex> lk(srv->pmu);
+                srv->pending.erase(std::find(srv->pending.begin(), srv->pending.end(), sl));
+            }
+            on_complete(sl, t);
+            any = true;
+        }
+        if (!any) std::this_thread::sleep_for(std::chrono::microseconds(5));
+    }
 }
 
 size_t esize(int vt) { return vt == FIP_VIS_F32 ? 4 : 2; }
@@ -145,6 +180,7 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
         std::lock_guard<std::recursive_mutex> lk(C->mu);
         auto* srv = new fip_server();
         srv->ring = ring;
+        srv->poller = std::thread(poll_loop, srv);
         size_t spare = hbm_budget_bytes - base;
         for (int k = 0; k < nstreams; ++k) {
             auto ss = std::make_unique<SubStream>();
@@ -203,7 +239,8 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
                 if ((e = sl->vr.ensure(vb)) != hipSuccess || (e = sl->vi.ensure(vb)) != hipSuccess ||
                     (e = sl->b.ensure(N * 8)) != hipSuccess || (e = sl->v.ensure(36)) != hipSuccess ||
                     (e = sl->host.ensure(2 * vb + N * 8 + 36)) != hipSuccess ||
-                    (e = sl->result.ensure(un * un * 4)) != hipSuccess)
+                    (e = sl->result.ensure(un * un * 4)) != hipSuccess ||
+                    (e = hipEventCreateWithFlags(&sl->done, hipEventDisableTiming)) != hipSuccess)
                     return fail(hip_error(e, "allocation (server slot)"));
                 ssp->slots.push_back(std::move(sl));
             }
@@ -302,12 +339,17 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
         std::lock_guard<std::mutex> lk2(srv->mu);
         srv->inflight += 1;
     }
-    hipError_t e = hipLaunchHostFunc(st, on_complete, sl);
+    hipError_t e = hipEventRecord(sl->done, st);
     if (e != hipSuccess) {
         std::lock_guard<std::mutex> lk2(srv->mu);
         srv->inflight -= 1;
-        return undo(hip_error(e, "hipLaunchHostFunc"));
+        return undo(hip_error(e, "hipEventRecord"));
     }
+    {
+        std::lock_guard<std::mutex> lk2(srv->pmu);
+        srv->pending.push_back(sl);
+    }
+    srv->pcv.notify_one();
 #undef SRV_HIP
     ss.submitted += 1;
     return FIP_OK;
@@ -317,13 +359,13 @@ int fip_server_drain(fip_server* srv)
 {
     if (!srv) return set_error(FIP_ERR_ARG, "null server");
     for (auto& ss : srv->streams) FIP_HIP(hipStreamSynchronize(ss->hs));
-    // Host functions may still be finishing on the runtime thread.
-    for (int spin = 0; spin < 100000; ++spin) {
+    // Every event has fired; wait for the completion thread to record them.
+    for (int spin = 0; spin < 200000; ++spin) {
         {
             std::lock_guard<std::mutex> lk(srv->mu);
             if (srv->inflight <= 0) return FIP_OK;
         }
-        std::this_thread::yield();
+        std::this_thread::sleep_for(std::chrono::microseconds(10));
     }
     return set_error(FIP_ERR_HIP, "completions did not arrive");
 }
@@ -372,6 +414,12 @@ void fip_server_destroy(fip_server* srv)
 {
     if (!srv) return;
     (void)fip_server_drain(srv);
+    {
+        std::lock_guard<std::mutex> lk(srv->pmu);
+        srv->stop = true;
+    }
+    srv->pcv.notify_one();
+    if (srv->poller.joinable()) srv->poller.join();
     for (auto& ss : srv->streams) {
         for (DevBuf* b : {&ss->rec, &ss->bkslot, &ss->sorted, &ss->hist, &ss->bstart, &ss->kt, &ss->pre, &ss->work,
                           &ss->img, &ss->meta, &ss->geom, &ss->win, &ss->out})
@@ -383,6 +431,7 @@ void fip_server_destroy(fip_server* srv)
             sl->v.release();
             sl->host.release();
             sl->result.release();
+            if (sl->done) (void)hipEventDestroy(sl->done);
         }
         if (ss->hs) (void)hipStreamDestroy(ss->hs);
     }

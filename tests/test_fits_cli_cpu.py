@@ -133,6 +133,10 @@ def test_cli_usage(cli):
     assert r.returncode == 1 and "usage" in r.stderr
     r = _run("--combined", "x.fits")
     assert r.returncode == 1
+    r = _run("a", "b", "--tau", "x")
+    assert r.returncode == 1 and "--tau" in r.stderr
+    r = _run("a", "b", "--tau", "-1")
+    assert r.returncode == 1 and "--tau" in r.stderr
 
 
 def _write_inputs(F, d, nbl=16, nsnap=3, naxis3=False):

@@ -317,6 +317,11 @@ def test_cli_end_to_end(fip, tmp_path):
         got = fip.fits_read(p[k])
         assert got.shape == (1, 16, 16)
         assert np.array_equal(got[0], ref)
+    r = _cli(p["vr"], p["vi"], p["b"], p["v"], w["image"], nbl, w["cell"], 3, w["tile"], p["o1"], "--tau", 0.999)
+    assert r.returncode == 0, r.stderr
+    listed = {tuple(int(x) for x in line.split()[2:5:2]) for line in r.stdout.splitlines()
+              if line.strip().startswith("tile row")}
+    assert listed == triggered(ref, 0.999) and len(listed) > 0
 
 
 def _write_combined(path, d, S):
