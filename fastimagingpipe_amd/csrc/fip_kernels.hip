@@ -345,27 +345,44 @@ __global__ __launch_bounds__(256) void k_scatter(const VisRec* __restrict__ rec,
 
 // ---------------------------------------------------------------- stage 1
 
-// Grid one visibility (or its Hermitian mirror) into uv column `col` for the
-// rows [lo, lo+CH) of the int64 band.  Taps follow FIPkernels.cu:213-243:
-// r in [ceil(p-4), floor(p+4)] clipped to [-g/2, g/2-1].
-__device__ __forceinline__ void grid_one(const VisRec& r, int col, bool mirror, int lo, int ch, int M,
-                                         float fscale, double* are, double* aim)
+// One visibility record prepared for a column: the column weight
+// ES((col - p2)/4)/2 (the Hermitian fold's 1/2) and the scale are folded
+// into its value; the row taps are applied per LDS band.
+struct ColRec {
+    int b1;
+    float f1, vr, vi;
+    bool mirror, live;
+};
+
+// Taps follow FIPkernels.cu:213-243: r in [ceil(p-4), floor(p+4)] clipped to
+// [-g/2, g/2-1].  A mirrored record contributes conj(v) at row -r.
+__device__ __forceinline__ ColRec prep_rec(const VisRec& r, int col, bool mirror, int M, float fscale)
 {
+    ColRec c;
+    c.b1 = r.b1;
+    c.f1 = r.f1;
+    c.mirror = mirror;
     const int lo2 = max(r.b2 - 4 + (r.f2 > 0.0f ? 1 : 0), -M);
     const int hi2 = min(r.b2 + 4, M - 1);
-    if (col < lo2 || col > hi2) return;
-    const float w2 = es_f(((float)(col - r.b2) - r.f2) * 0.25f) * 0.5f * fscale;
-    const float vr = r.vr * w2;
-    const float vi = (mirror ? -r.vi : r.vi) * w2;
-    const int lo1 = max(r.b1 - 4 + (r.f1 > 0.0f ? 1 : 0), -M);
-    const int hi1 = min(r.b1 + 4, M - 1);
+    c.live = (col >= lo2 && col <= hi2);
+    const float w2 = c.live ? es_f(((float)(col - r.b2) - r.f2) * 0.25f) * 0.5f * fscale : 0.0f;
+    c.vr = r.vr * w2;
+    c.vi = (mirror ? -r.vi : r.vi) * w2;
+    return c;
+}
+
+// Row taps of one prepared record inside the band [lo, lo+ch).
+__device__ __forceinline__ void grid_taps(const ColRec& c, int lo, int ch, int M, double* are, double* aim)
+{
+    const int lo1 = max(c.b1 - 4 + (c.f1 > 0.0f ? 1 : 0), -M);
+    const int hi1 = min(c.b1 + 4, M - 1);
     for (int t1 = lo1; t1 <= hi1; ++t1) {
-        const int row = mirror ? ((t1 == -M) ? -M : -t1) : t1;
+        const int row = c.mirror ? ((t1 == -M) ? -M : -t1) : t1;
         const int q = row - lo;
         if ((unsigned)q < (unsigned)ch) {
-            const float w1 = es_f(((float)(t1 - r.b1) - r.f1) * 0.25f);
-            atomicAdd(&are[q], rint((double)(vr * w1)));
-            atomicAdd(&aim[q], rint((double)(vi * w1)));
+            const float w1 = es_f(((float)(t1 - c.b1) - c.f1) * 0.25f);
+            atomicAdd(&are[q], rint((double)(c.vr * w1)));
+            atomicAdd(&aim[q], rint((double)(c.vi * w1)));
         }
     }
 }
@@ -399,31 +416,48 @@ __device__ __forceinline__ void stage1_column(int s, int c, const VisRec* __rest
     const int d0 = bst[min(max(cc - 4 + M, 0), G - 1)], d1 = bst[min(max(cc + 4 + M, 0), G - 1) + 1];
     const int m0 = bst[min(max(cm - 4 + M, 0), G - 1)], m1 = bst[min(max(cm + 4 + M, 0), G - 1) + 1];
     const int rlo = -m.r1max, rhi = min(m.r1max, M - 1);
+    // Direct records (buckets cc-4..cc+4) then mirrored ones (cm-4..cm+4) in
+    // one numbering; each thread holds KR of them in registers, so a column
+    // with at most KR*NT records is read from memory once for all bands.
+    constexpr int KR = 4;
+    const int nd = d1 - d0, ntot = nd + (m1 - m0);
+    ColRec cr[KR];
+    auto load = [&](int base) {
+        VisRec r[KR];
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int n = base + k * NT + t;
+            if (n < ntot) r[k] = vs[n < nd ? d0 + n : m0 + (n - nd)];
+        }
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int n = base + k * NT + t;
+            if (n < ntot)
+                cr[k] = prep_rec(r[k], n < nd ? cc : cm, n >= nd, M, fscale);
+            else
+                cr[k].live = false;
+        }
+    };
+    auto band = [&](int lo) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k)
+            if (cr[k].live) grid_taps(cr[k], lo, CH, M, are, aim);
+    };
+    const bool resident = ntot <= KR * NT;
+    if (resident) load(0);
     for (int lo = rlo; lo <= rhi; lo += CH) {
         for (int i = t; i < CH; i += NT) {
             are[i] = 0.0;
             aim[i] = 0.0;
         }
         __syncthreads();
-        // Four visibilities per thread per iteration: their loads are issued
-        // together so HBM/L2 latency is paid once per four.
-        for (int n = d0 + t; n < d1; n += 4 * NT) {
-            VisRec r[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (n + u * NT < d1) r[u] = vs[n + u * NT];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (n + u * NT < d1) grid_one(r[u], cc, false, lo, CH, M, fscale, are, aim);
-        }
-        for (int n = m0 + t; n < m1; n += 4 * NT) {
-            VisRec r[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (n + u * NT < m1) r[u] = vs[n + u * NT];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (n + u * NT < m1) grid_one(r[u], cm, true, lo, CH, M, fscale, are, aim);
+        if (resident) {
+            band(lo);
+        } else {
+            for (int base = 0; base < ntot; base += KR * NT) {
+                load(base);
+                band(lo);
+            }
         }
         __syncthreads();
         const int nrow = min(CH, rhi - lo + 1);
@@ -555,20 +589,34 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
     __syncthreads();
     // C2R split: Z[k] = E[k] + i O[k], E = K[k] + K~[k+M], O = (K[k] - K~[k+M]) e^{2 pi i k/G},
     // K~[k+M] = conj(K[M-k]) (k > 0), K[M] (k = 0).  Pairs (p, M-p) are disjoint.
-    for (int p = t; p <= M / 2; p += NT) {
-        const float2 a = z[lds_pad(p)];
-        const float2 b = z[lds_pad(M - p)];
-        if (p == 0) {
-            z[lds_pad(0)] = cadd(cadd(a, b), cmul_i(csub(a, b)));
-        } else {
-            const float2 kp = cconj(b);   // K~[p+M]
-            const float2 zp = cadd(cadd(a, kp), cmul_i(cmul(csub(a, kp), wpost[p])));
-            if (p != M - p) {
-                const float2 km = cconj(a);   // K~[M-p+M]
-                const float2 zm = cadd(cadd(b, km), cmul_i(cmul(csub(b, km), wpost[M - p])));
-                z[lds_pad(M - p)] = zm;
+    // wpost[M - p] = -conj(wpost[p]) (e^{2 pi i (M-p)/G} with G = 2M); the
+    // loads of four pairs per thread are issued together.
+    for (int p0 = t; p0 <= M / 2; p0 += 4 * NT) {
+        float2 wv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = p0 + u * NT;
+            wv[u] = (p <= M / 2) ? wpost[p] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = p0 + u * NT;
+            if (p > M / 2) break;
+            const float2 a = z[lds_pad(p)];
+            const float2 b = z[lds_pad(M - p)];
+            if (p == 0) {
+                z[lds_pad(0)] = cadd(cadd(a, b), cmul_i(csub(a, b)));
+            } else {
+                const float2 kp = cconj(b);   // K~[p+M]
+                const float2 zp = cadd(cadd(a, kp), cmul_i(cmul(csub(a, kp), wv[u])));
+                if (p != M - p) {
+                    const float2 km = cconj(a);   // K~[M-p+M]
+                    const float2 wm = make_float2(-wv[u].x, wv[u].y);
+                    const float2 zm = cadd(cadd(b, km), cmul_i(cmul(csub(b, km), wm)));
+                    z[lds_pad(M - p)] = zm;
+                }
+                z[lds_pad(p)] = zp;
             }
-            z[lds_pad(p)] = zp;
         }
     }
     __syncthreads();
