@@ -371,56 +371,98 @@ __device__ __forceinline__ ColRec prep_rec(const VisRec& r, int col, bool mirror
     return c;
 }
 
-// Row taps of one prepared record inside the band [lo, lo+ch).
-__device__ __forceinline__ void grid_taps(const ColRec& c, int lo, int ch, int M, double* are, double* aim)
+// Grid rows are grouped in at most kScaleBlocks blocks of at least 16 rows
+// (by the index r mod G of row r); each block has its own power-of-two scale.
+constexpr int kScaleBlocks = 96;
+
+__device__ __forceinline__ int tap_row(const ColRec& c, int t1, int M)
+{
+    return c.mirror ? ((t1 == -M) ? -M : -t1) : t1;
+}
+
+// Row taps of one prepared record, accumulated as integers into the LDS
+// column (re, im interleaved; row r at padded index of r mod G), each scaled
+// by its block's power of two.
+__device__ __forceinline__ void grid_taps(const ColRec& c, int M, int G, int rbs, const float* sc, int* col)
 {
     const int lo1 = max(c.b1 - 4 + (c.f1 > 0.0f ? 1 : 0), -M);
     const int hi1 = min(c.b1 + 4, M - 1);
     for (int t1 = lo1; t1 <= hi1; ++t1) {
-        const int row = c.mirror ? ((t1 == -M) ? -M : -t1) : t1;
-        const int q = row - lo;
-        if ((unsigned)q < (unsigned)ch) {
-            const float w1 = es_f(((float)(t1 - c.b1) - c.f1) * 0.25f);
-            atomicAdd(&are[q], rint((double)(c.vr * w1)));
-            atomicAdd(&aim[q], rint((double)(c.vi * w1)));
-        }
+        const int row = tap_row(c, t1, M);
+        const int idx = row < 0 ? row + G : row;
+        const int q = 2 * lds_pad(idx);
+        const float w1 = es_f(((float)(t1 - c.b1) - c.f1) * 0.25f);
+        const float k = sc[idx / rbs];
+        atomicAdd(&col[q], (int)rintf((c.vr * w1) * k));
+        atomicAdd(&col[q + 1], (int)rintf((c.vi * w1) * k));
     }
 }
 
+// Adds the record's magnitude bound (units of amax / 4096, rounded up) to the
+// mass of every scale block its taps touch (at most two).
+__device__ __forceinline__ void tap_mass(const ColRec& c, int M, int G, int rbs, float qs, unsigned* mass)
+{
+    const int lo1 = max(c.b1 - 4 + (c.f1 > 0.0f ? 1 : 0), -M);
+    const int hi1 = min(c.b1 + 4, M - 1);
+    if (lo1 > hi1) return;
+    const unsigned q = (unsigned)ceilf(fmaxf(fabsf(c.vr), fabsf(c.vi)) * qs);
+    const int ra = tap_row(c, lo1, M), rb = tap_row(c, hi1, M);
+    const int ba = (ra < 0 ? ra + G : ra) / rbs, bb = (rb < 0 ? rb + G : rb) / rbs;
+    atomicAdd(&mass[ba], q);
+    if (bb != ba) atomicAdd(&mass[bb], q);
+}
+
+// Record ranges of column c: direct buckets cc-4..cc+4 and mirrored buckets
+// cm-4..cm+4 (x, y: direct [x, y); z, w: mirrored [z, w)).
+template <int G>
+__device__ __forceinline__ int4 column_ranges(const int* __restrict__ bst, int c)
+{
+    constexpr int M = G / 2;
+    const int cc = (c == M) ? -M : c;
+    const int cm = (c == 0) ? 0 : ((c == M) ? -M : -c);
+    int4 r;
+    r.x = bst[min(max(cc - 4 + M, 0), G - 1)];
+    r.y = bst[min(max(cc + 4 + M, 0), G - 1) + 1];
+    r.z = bst[min(max(cm - 4 + M, 0), G - 1)];
+    r.w = bst[min(max(cm + 4 + M, 0), G - 1) + 1];
+    return r;
+}
+
 template <int G, int CH>
-__device__ __forceinline__ void stage1_column(int s, int c, const VisRec* __restrict__ sorted,
-                                              const int* __restrict__ bstart, const SnapMeta& m,
+__device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRec* __restrict__ sorted,
+                                              const SnapMeta& m,
                                               const float2* __restrict__ tw, const float* __restrict__ corr,
                                               float2* __restrict__ kt, size_t stride, int I, float2* fb,
-                                              double* are, double* aim)
+                                              unsigned* mass, float* sc, float* isc)
 {
     constexpr int NT = FftPlan<G>::nt;
     constexpr int M = G / 2;
     const int t = threadIdx.x;
-    for (int i = t; i < G; i += NT) fb[lds_pad(i)] = make_float2(0.0f, 0.0f);
 
-    // Exact-integer accumulation: contributions are scaled by 2^k and rounded
-    // to integers; every cell sum is bounded by ngrid * maxv, so with
-    // 2^k * ngrid * maxv <= 2^52 all partial sums are exact doubles and the
-    // result does not depend on the order of the LDS atomics.
-    const float maxv = __uint_as_float(m.maxv_bits);
-    int e = 0;
-    frexpf(maxv * (float)max(m.ngrid, 1), &e);
-    const float fscale = ldexpf(1.0f, 52 - e);
-    const double iscale = ldexp(1.0, e - 52);
-
-    const int* bst = bstart + (size_t)s * (G + 1);
     const VisRec* vs = sorted + (size_t)s * stride;
     const int cc = (c == M) ? -M : c;                        // direct column (centred)
     const int cm = (c == 0) ? 0 : ((c == M) ? -M : -c);      // mirrored column
-    const int d0 = bst[min(max(cc - 4 + M, 0), G - 1)], d1 = bst[min(max(cc + 4 + M, 0), G - 1) + 1];
-    const int m0 = bst[min(max(cm - 4 + M, 0), G - 1)], m1 = bst[min(max(cm + 4 + M, 0), G - 1) + 1];
-    const int rlo = -m.r1max, rhi = min(m.r1max, M - 1);
-    // Direct records (buckets cc-4..cc+4) then mirrored ones (cm-4..cm+4) in
-    // one numbering; each thread holds KR of them in registers, so a column
-    // with at most KR*NT records is read from memory once for all bands.
-    constexpr int KR = 4;
+    const int d0 = rg.x, d1 = rg.y, m0 = rg.z, m1 = rg.w;
     const int nd = d1 - d0, ntot = nd + (m1 - m0);
+
+    // Exact integer accumulation with one power-of-two scale per block of
+    // rows.  Pass 1 adds, per block, an upper bound of the magnitudes of the
+    // records whose taps reach it (integers, so the bound does not depend on
+    // the order of the LDS atomics); the scale keeps every int32 cell sum of
+    // the block below 2^29 in magnitude.  Pass 2 accumulates the scaled,
+    // rounded contributions; sums of integers are exact, so the column does
+    // not depend on the order of the atomics either.
+    const float maxv = __uint_as_float(m.maxv_bits);
+    const float amax = fmaxf(0.5f * maxv, 1e-30f);   // |v| ES / 2 <= amax
+    const float qs = 4096.0f / amax;
+    // At least 16 rows per block, so the 9 taps of a record reach at most two
+    // blocks (two ends of a contiguous run of r mod G); at most kScaleBlocks.
+    constexpr int RBS = (G / kScaleBlocks > 16) ? G / kScaleBlocks : 16;
+    static_assert(G % RBS == 0 && G / RBS <= kScaleBlocks, "scale blocks");
+
+    // Direct records (buckets cc-4..cc+4) then mirrored ones (cm-4..cm+4) in
+    // one numbering; each thread holds KR of them in registers at a time.
+    constexpr int KR = 4;
     ColRec cr[KR];
     auto load = [&](int base) {
         VisRec r[KR];
@@ -433,42 +475,51 @@ __device__ __forceinline__ void stage1_column(int s, int c, const VisRec* __rest
         for (int k = 0; k < KR; ++k) {
             const int n = base + k * NT + t;
             if (n < ntot)
-                cr[k] = prep_rec(r[k], n < nd ? cc : cm, n >= nd, M, fscale);
+                cr[k] = prep_rec(r[k], n < nd ? cc : cm, n >= nd, M, 1.0f);
             else
                 cr[k].live = false;
         }
     };
-    auto band = [&](int lo) {
-#pragma unroll
-        for (int k = 0; k < KR; ++k)
-            if (cr[k].live) grid_taps(cr[k], lo, CH, M, are, aim);
-    };
     const bool resident = ntot <= KR * NT;
     if (resident) load(0);
-    for (int lo = rlo; lo <= rhi; lo += CH) {
-        for (int i = t; i < CH; i += NT) {
-            are[i] = 0.0;
-            aim[i] = 0.0;
-        }
-        __syncthreads();
-        if (resident) {
-            band(lo);
-        } else {
-            for (int base = 0; base < ntot; base += KR * NT) {
-                load(base);
-                band(lo);
-            }
-        }
-        __syncthreads();
-        const int nrow = min(CH, rhi - lo + 1);
-        for (int i = t; i < nrow; i += NT) {
-            const int row = lo + i;
-            const float re = (float)(are[i] * iscale);
-            const float im = (float)(aim[i] * iscale);
-            fb[lds_pad(row < 0 ? row + G : row)] = make_float2(re, im);
-        }
-        __syncthreads();
+    int* col = reinterpret_cast<int*>(fb);
+    for (int i = t; i < G; i += NT) fb[lds_pad(i)] = make_float2(0.0f, 0.0f);   // integer zero too
+    for (int i = t; i < kScaleBlocks; i += NT) mass[i] = 0u;
+    __syncthreads();
+    for (int base = 0; base < ntot; base += KR * NT) {
+        if (!resident) load(base);
+#pragma unroll
+        for (int k = 0; k < KR; ++k)
+            if (cr[k].live) tap_mass(cr[k], M, G, RBS, qs, mass);
     }
+    __syncthreads();
+    for (int i = t; i < kScaleBlocks; i += NT) {
+        // bound = mass * amax / 4096 (+ one unit per record of rounding slack
+        // is already inside the ceil); 2^e > bound.
+        const float bound = (float)mass[i] * (amax / 4096.0f) * 1.0001f;
+        int e = 0;
+        frexpf(fmaxf(bound, 1e-30f), &e);
+        const int k = min(29 - e, 120);   // empty blocks: any finite scale
+        sc[i] = ldexpf(1.0f, k);
+        isc[i] = ldexpf(1.0f, -k);
+    }
+    __syncthreads();
+    for (int base = 0; base < ntot; base += KR * NT) {
+        if (!resident) load(base);
+#pragma unroll
+        for (int k = 0; k < KR; ++k)
+            if (cr[k].live) grid_taps(cr[k], M, G, RBS, sc, col);
+    }
+    __syncthreads();
+    const int rlo = -m.r1max, rhi = min(m.r1max, M - 1);
+    for (int i = rlo + t; i <= rhi; i += NT) {
+        const int idx = i < 0 ? i + G : i;
+        const int q = lds_pad(idx);
+        const int re = col[2 * q], im = col[2 * q + 1];
+        const float k = isc[idx / RBS];
+        fb[q] = make_float2((float)re * k, (float)im * k);
+    }
+    __syncthreads();
     fft_inverse<G>(fb, tw, t);
     // Crop y in [-h, h) and apply the row correction 1/(c(|y|) norm).
     const int h = I / 2;
@@ -502,15 +553,19 @@ __global__ __launch_bounds__(FftPlan<G>::nt, 2) void k_stage1(const VisRec* __re
                                                             float2* __restrict__ kt, size_t stride, int I)
 {
     __shared__ float2 fb[lds_len(G)];
-    __shared__ double are[CH], aim[CH];
+    __shared__ unsigned mass[kScaleBlocks];
+    __shared__ float sc[kScaleBlocks], isc[kScaleBlocks];
     // Column-major over snapshots: the heavily populated low columns of
     // every snapshot are dispatched first, so none of them ends the launch.
     const int S = (int)(gridDim.x / (G / 2 + 1));
     const int s = (int)(blockIdx.x % S);
     const int c = (int)(blockIdx.x / S);
+    // The bucket ranges do not depend on the metadata: both loads are in
+    // flight together.
+    const int4 rg = column_ranges<G>(bstart + (size_t)s * (G + 1), c);
     const SnapMeta m = meta[s];
     if (c > m.rmax) return;
-    stage1_column<G, CH>(s, c, sorted, bstart, m, tw, corr, kt, stride, I, fb, are, aim);
+    stage1_column<G, CH>(s, c, rg, sorted, m, tw, corr, kt, stride, I, fb, mass, sc, isc);
 }
 
 // Persistent form: a fixed grid pulls (snapshot, column) items from a
@@ -522,7 +577,8 @@ __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
     int I, int S, unsigned* work)
 {
     __shared__ float2 fb[lds_len(G)];
-    __shared__ double are[CH], aim[CH];
+    __shared__ unsigned mass[kScaleBlocks];
+    __shared__ float sc[kScaleBlocks], isc[kScaleBlocks];
     __shared__ unsigned item;
     constexpr unsigned per = G / 2 + 1;
     const unsigned total = (unsigned)S * per;
@@ -533,9 +589,10 @@ __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
         __syncthreads();
         if (it >= total) break;
         const int s = (int)(it % (unsigned)S), c = (int)(it / (unsigned)S);
+        const int4 rg = column_ranges<G>(bstart + (size_t)s * (G + 1), c);
         const SnapMeta m = meta[s];
         if (c > m.rmax) continue;
-        stage1_column<G, CH>(s, c, sorted, bstart, m, tw, corr, kt, stride, I, fb, are, aim);
+        stage1_column<G, CH>(s, c, rg, sorted, m, tw, corr, kt, stride, I, fb, mass, sc, isc);
         __syncthreads();
     }
 }
@@ -718,6 +775,7 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int R0 = (tile / ntx) * TS, C0 = (tile % ntx) * TS;
     const SnapGeom g = geom[s];
+    const SnapMeta m = meta[s];
     const int h = I / 2;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     if (tid == 0) {
@@ -779,7 +837,6 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     __syncthreads();
     const int i0 = bb[0], i1 = bb[1], j0 = bb[2], j1 = bb[3];
     const int H = j1 - j0 + 1;
-    const SnapMeta m = meta[s];
     // 2^k * max contribution <= 2^44: sums of up to 256 contributions stay
     // exact integers below 2^52.
     int e = 0;
