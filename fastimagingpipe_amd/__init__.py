@@ -282,13 +282,13 @@ def reset_stage_times() -> None:
 def autotune(image_size: int, num_baselines: int, cell_size: float, reps: int = 3):
     ch = np.zeros(4, np.int32)
     _check(lib.fip_autotune(image_size, num_baselines, cell_size, reps, _ptr(ch)))
-    return {"stage1_chunk": int(ch[0]), "stage2_rows": int(ch[1]), "splat_tile": int(ch[2]),
+    return {"stage1_records": int(ch[0]), "stage2_rows": int(ch[1]), "splat_tile": int(ch[2]),
             "persistent": int(ch[3])}
 
 
-def set_tuning(image_size: int, stage1_chunk: int, stage2_rows: int, splat_tile: int,
+def set_tuning(image_size: int, stage1_records: int, stage2_rows: int, splat_tile: int,
                persistent: int) -> None:
-    ch = np.array([stage1_chunk, stage2_rows, splat_tile, persistent], np.int32)
+    ch = np.array([stage1_records, stage2_rows, splat_tile, persistent], np.int32)
     _check(lib.fip_set_tuning(image_size, _ptr(ch)))
 
 

@@ -415,7 +415,7 @@ int fip_batch_device(const void* vis_re, const void* vis_im, int vis_type, const
         key.tile = unit_size;
         key.vis_type = vis_type;
         std::memcpy(&key.cell_bits, &cell_size, 4);
-        key.t0 = tu.stage1_chunk;
+        key.t0 = tu.stage1_records;
         key.t1 = tu.stage2_rows;
         key.t2 = tu.splat_tile;
         key.t3 = tu.persistent;
@@ -535,7 +535,7 @@ int fip_set_tuning(size_t image_size, const int* choice)
     if (int rc = Context::get(&C)) return rc;
     std::lock_guard<std::recursive_mutex> lk(C->mu);
     Tuning t;
-    t.stage1_chunk = choice[0];
+    t.stage1_records = choice[0];
     t.stage2_rows = choice[1];
     t.splat_tile = choice[2];
     t.persistent = choice[3];
@@ -611,19 +611,19 @@ int fip_autotune(size_t image_size, size_t num_baselines, float cell_size, int r
         };
         Tuning best = C->tuning(image_size);
         double best_s1 = 1e300, best_s2 = 1e300, best_sp = 1e300;
-        const int chunks[2] = {1, 1 << 30};
+        const int records[2] = {4, 8};
         const int rows[3] = {1, 2, 4};
         const int tiles[2] = {32, 64};
         double tm[kStCount];
         for (int ci = 0; ci < 2; ++ci) {
             for (int pers = 0; pers < 2; ++pers) {
                 Tuning t = best;
-                t.stage1_chunk = chunks[ci];
+                t.stage1_records = records[ci];
                 t.persistent = pers;
                 if (int rc = measure(t, tm)) return rc;
                 if (tm[kStGridFft] < best_s1) {
                     best_s1 = tm[kStGridFft];
-                    best.stage1_chunk = t.stage1_chunk;
+                    best.stage1_records = t.stage1_records;
                     best.persistent = t.persistent;
                 }
             }
@@ -651,7 +651,7 @@ int fip_autotune(size_t image_size, size_t num_baselines, float cell_size, int r
         C->profiling = was;
         C->set_tuning(image_size, best);
         if (choice) {
-            choice[0] = best.stage1_chunk;
+            choice[0] = best.stage1_records;
             choice[1] = best.stage2_rows;
             choice[2] = best.splat_tile;
             choice[3] = best.persistent;

@@ -199,7 +199,7 @@ Tuning Context::tuning(size_t image) const
     auto it = tunings_.find(image);
     if (it != tunings_.end()) return it->second;
     Tuning t;
-    t.stage1_chunk = 1;         // the small LDS band: two workgroups per CU
+    t.stage1_records = 4;         // gridding records per thread in registers
     t.stage2_rows = 2;
     t.splat_tile = 64;
     t.persistent = 0;

@@ -94,7 +94,7 @@ struct WorkView {
 
 // Tunable launch parameters (the autotuner picks among these).
 struct Tuning {
-    int stage1_chunk;   // int64 gridding rows held in LDS per pass
+    int stage1_records;   // gridding records per thread held in registers (2, 4, 8)
     int stage2_rows;    // image rows per stage-2 workgroup (1, 2, 4)
     int splat_tile;     // target tile edge of the splat (32 or 64)
     int persistent;     // stage 1 as a persistent work-queue kernel

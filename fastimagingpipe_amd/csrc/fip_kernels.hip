@@ -428,7 +428,7 @@ __device__ __forceinline__ int4 column_ranges(const int* __restrict__ bst, int c
     return r;
 }
 
-template <int G, int CH>
+template <int G, int KR>
 __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRec* __restrict__ sorted,
                                               const SnapMeta& m,
                                               const float2* __restrict__ tw, const float* __restrict__ corr,
@@ -462,7 +462,6 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
 
     // Direct records (buckets cc-4..cc+4) then mirrored ones (cm-4..cm+4) in
     // one numbering; each thread holds KR of them in registers at a time.
-    constexpr int KR = 4;
     ColRec cr[KR];
     auto load = [&](int base) {
         VisRec r[KR];
@@ -544,7 +543,7 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
     }
 }
 
-template <int G, int CH>
+template <int G, int KR>
 __global__ __launch_bounds__(FftPlan<G>::nt, 2) void k_stage1(const VisRec* __restrict__ sorted,
                                                             const int* __restrict__ bstart,
                                                             const SnapMeta* __restrict__ meta,
@@ -565,12 +564,12 @@ __global__ __launch_bounds__(FftPlan<G>::nt, 2) void k_stage1(const VisRec* __re
     const int4 rg = column_ranges<G>(bstart + (size_t)s * (G + 1), c);
     const SnapMeta m = meta[s];
     if (c > m.rmax) return;
-    stage1_column<G, CH>(s, c, rg, sorted, m, tw, corr, kt, stride, I, fb, mass, sc, isc);
+    stage1_column<G, KR>(s, c, rg, sorted, m, tw, corr, kt, stride, I, fb, mass, sc, isc);
 }
 
 // Persistent form: a fixed grid pulls (snapshot, column) items from a
 // counter until all S*(M+1) items are claimed; every workgroup exits.
-template <int G, int CH>
+template <int G, int KR>
 __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
     const VisRec* __restrict__ sorted, const int* __restrict__ bstart, const SnapMeta* __restrict__ meta,
     const float2* __restrict__ tw, const float* __restrict__ corr, float2* __restrict__ kt, size_t stride,
@@ -592,7 +591,7 @@ __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
         const int4 rg = column_ranges<G>(bstart + (size_t)s * (G + 1), c);
         const SnapMeta m = meta[s];
         if (c > m.rmax) continue;
-        stage1_column<G, CH>(s, c, rg, sorted, m, tw, corr, kt, stride, I, fb, mass, sc, isc);
+        stage1_column<G, KR>(s, c, rg, sorted, m, tw, corr, kt, stride, I, fb, mass, sc, isc);
         __syncthreads();
     }
 }
@@ -1092,7 +1091,7 @@ hipError_t launch_scatter(const WorkView& w, const PlanTables& p, int S, int nma
     return hipGetLastError();
 }
 
-template <int G, int CH>
+template <int G, int KR>
 static void stage1_dispatch(const WorkView& w, const PlanTables& p, int S, bool persistent, hipStream_t st)
 {
     constexpr int NT = FftPlan<G>::nt;
@@ -1100,35 +1099,26 @@ static void stage1_dispatch(const WorkView& w, const PlanTables& p, int S, bool 
         int dev = 0, ncu = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage1_persistent<G, CH>, NT, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage1_persistent<G, KR>, NT, 0);
         const int nb = max(1, ncu * max(per_cu, 1));
-        hipLaunchKernelGGL((k_stage1_persistent<G, CH>), dim3(nb), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
+        hipLaunchKernelGGL((k_stage1_persistent<G, KR>), dim3(nb), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
                            p.tw_g, p.corr, w.kt, w.stride, p.I, S, w.work);
     } else {
-        hipLaunchKernelGGL((k_stage1<G, CH>), dim3((G / 2 + 1) * S), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
+        hipLaunchKernelGGL((k_stage1<G, KR>), dim3((G / 2 + 1) * S), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
                            p.tw_g, p.corr, w.kt, w.stride, p.I);
     }
 }
 
-// LDS int64 band (rows per gridding pass) for each transform length; the
-// second value is the small-chunk variant that lets two workgroups share a CU.
-template <int G> struct ChunkOpts;
-template <> struct ChunkOpts<96> { static constexpr int big = 96, small = 48; };
-template <> struct ChunkOpts<192> { static constexpr int big = 192, small = 96; };
-template <> struct ChunkOpts<384> { static constexpr int big = 384, small = 192; };
-template <> struct ChunkOpts<768> { static constexpr int big = 768, small = 384; };
-template <> struct ChunkOpts<1536> { static constexpr int big = 1536, small = 768; };
-template <> struct ChunkOpts<3072> { static constexpr int big = 3072, small = 1536; };
-template <> struct ChunkOpts<6144> { static constexpr int big = 3584, small = 1792; };
-template <> struct ChunkOpts<12288> { static constexpr int big = 3072, small = 1536; };
-
+// Records per thread held in registers while a column is gridded.
 template <int G>
 static void stage1_pick(const WorkView& w, const PlanTables& p, int S, const Tuning& tu, hipStream_t st)
 {
-    if (tu.stage1_chunk <= ChunkOpts<G>::small)
-        stage1_dispatch<G, ChunkOpts<G>::small>(w, p, S, tu.persistent != 0, st);
+    if (tu.stage1_records >= 8)
+        stage1_dispatch<G, 8>(w, p, S, tu.persistent != 0, st);
+    else if (tu.stage1_records <= 2)
+        stage1_dispatch<G, 2>(w, p, S, tu.persistent != 0, st);
     else
-        stage1_dispatch<G, ChunkOpts<G>::big>(w, p, S, tu.persistent != 0, st);
+        stage1_dispatch<G, 4>(w, p, S, tu.persistent != 0, st);
 }
 
 hipError_t launch_stage1(const WorkView& w, const PlanTables& p, int S, const Tuning& tu, hipStream_t st)

@@ -164,10 +164,10 @@ int fip_stage_times(double *ms, long *counts, int n_stages);
 int fip_stage_times_reset(void);
 
 /* ---- launch tuning and autotuner ---------------------------------------- */
-/* Measure candidate launch configurations (gridding LDS band, rows per C2R
- * workgroup, splat tile, persistent gridding) on synthetic data of the given
- * shape, keep the fastest for that image size, and write it to *choice
- * (4 ints: chunk, rows, tile, persistent; may be NULL). */
+/* Measure candidate launch configurations (gridding records per thread,
+ * rows per C2R workgroup, splat tile, persistent gridding) on synthetic data
+ * of the given shape, keep the fastest for that image size, and write it to
+ * *choice (4 ints: records, rows, tile, persistent; may be NULL). */
 int fip_autotune(size_t image_size, size_t num_baselines, float cell_size,
                  int reps, int *choice);
 /* Force a launch configuration for an image size (choice as above). */

@@ -374,3 +374,25 @@ def test_cli_combined_all_windows(fip, tmp_path):
                          w["tile"])
     assert np.array_equal(cube, ref)
     assert np.abs(one - ref[0]).max() < 1e-6
+
+
+def test_launch_tunings_give_identical_results(fip):
+    """Gridding sums, C2R rows and splat tiles are exact or independent per
+    row, so every launch configuration yields the same bits."""
+    w = SMALL
+    d = synth_workload(fip, w, seed=96)
+    args = (d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nbl"], w["image"], 3, w["cell"], w["tile"])
+    fip.set_tuning(w["image"], 4, 2, 64, 0)
+    ref = fip.fipipe(*args)
+    try:
+        for tu in [(8, 1, 32, 1), (2, 4, 64, 1), (4, 2, 32, 0)]:
+            fip.set_tuning(w["image"], *tu)
+            got = fip.fipipe(*args)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), tu
+        choice = fip.autotune(w["image"], d["nbl"], w["cell"], reps=1)
+        print("autotune choice at 256^2:", choice)
+        assert choice["stage1_records"] in (2, 4, 8) and choice["splat_tile"] in (32, 64)
+        got = fip.fipipe(*args)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    finally:
+        fip.set_tuning(w["image"], 4, 2, 64, 0)
