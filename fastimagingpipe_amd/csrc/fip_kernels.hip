@@ -763,10 +763,10 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     constexpr int A = TS + 2;             // apron: targets -1 .. TS
     constexpr int UB = 8;                 // sources in flight per lane
     constexpr int CHK = 512;              // runs of 64 numbered sources per window
-    __shared__ double acc[A * A];
+    __shared__ __align__(16) double acc[A * A];
     __shared__ int rlo[ROW_CAP], rcnt[ROW_CAP], rofs[ROW_CAP + 1];
     __shared__ unsigned char crow[CHK];   // row in which each run of 64 starts
-    __shared__ double qb[4];
+    __shared__ double qb[6];
     __shared__ int bb[4];
     __shared__ unsigned shm[NT / 64];
     const int s = blockIdx.y;
@@ -827,12 +827,18 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
         qb[1] = qh1;
         qb[2] = ql2;
         qb[3] = qh2;
+        qb[4] = fabs(g.m00) > 1e-12 ? 1.0 / g.m00 : 0.0;
+        qb[5] = fabs(g.m10) > 1e-12 ? 1.0 / g.m10 : 0.0;
         bb[0] = i0;
         bb[1] = i1;
         bb[2] = j0;
         bb[3] = j1;
     }
-    for (int q = tid; q < A * A; q += NT) acc[q] = 0.0;
+    {
+        // 16-byte stores (A * A is even and acc is 16-byte aligned).
+        double2* a2 = reinterpret_cast<double2*>(acc);
+        for (int q = tid; q < A * A / 2; q += NT) a2[q] = make_double2(0.0, 0.0);
+    }
     __syncthreads();
     const int i0 = bb[0], i1 = bb[1], j0 = bb[2], j1 = bb[3];
     const int H = j1 - j0 + 1;
@@ -851,12 +857,14 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
         for (int r = tid; r < nrow; r += NT) {
             const double b = (double)(j0 + rb + r - h);
             double alo = -1e300, ahi = 1e300;
-            const double mA[2] = {g.m00, g.m10}, mB[2] = {g.m01, g.m11};
+            const double mA[2] = {g.m00, g.m10}, mB[2] = {g.m01, g.m11}, iA[2] = {qb[4], qb[5]};
             const double lo[2] = {ql1, ql2}, hi[2] = {qh1, qh2};
             for (int k = 0; k < 2; ++k) {
                 const double cst = mB[k] * b;
                 if (fabs(mA[k]) > 1e-12) {
-                    double x = (lo[k] - cst) / mA[k], y = (hi[k] - cst) / mA[k];
+                    // The range is widened by one pixel below, so a rounded
+                    // reciprocal is as good as a division here.
+                    double x = (lo[k] - cst) * iA[k], y = (hi[k] - cst) * iA[k];
                     if (x > y) {
                         const double t = x;
                         x = y;
@@ -955,11 +963,14 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     }
     float* out = img + (size_t)s * I * I;
     float lmax = 0.0f;
-    for (int q = tid; q < TS * TS; q += NT) {
-        const int rr = q / TS, cc = q % TS;
-        const float v = (float)(acc[(rr + 1) * A + cc + 1] * iscale);
-        out[(size_t)(R0 + rr) * I + C0 + cc] = v;
-        lmax = fmaxf(lmax, v);
+    // Four consecutive pixels per thread per pass, one 16-byte store.
+    for (int q = tid; q < TS * TS / 4; q += NT) {
+        const int rr = q / (TS / 4), cc = 4 * (q % (TS / 4));
+        const double* a = acc + (rr + 1) * A + cc + 1;
+        const float4 v = make_float4((float)(a[0] * iscale), (float)(a[1] * iscale), (float)(a[2] * iscale),
+                                     (float)(a[3] * iscale));
+        *reinterpret_cast<float4*>(out + (size_t)(R0 + rr) * I + C0 + cc) = v;
+        lmax = fmaxf(lmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
     }
     const unsigned bmx = block_max_u<NT>(__float_as_uint(lmax), shm);
     if (tid == 0) atomicMax(&meta[s].maxd_bits, bmx);
