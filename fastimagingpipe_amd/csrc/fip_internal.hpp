@@ -49,7 +49,7 @@ struct SnapGeom {
     double curv;                 // bound on the SIN map's curvature (1/px), <0: unbounded
     double kappa;                // bound on |grad| of the SIN shift over the image
     int sin_on;                  // xi^2 + eta^2 != 0
-    int phased;                  // sources 3 apart map >= 2*sqrt(2) apart: phased float splat is exact-order
+    int series;                  // every direction cosine has r^2 <= 0.01: n' by its power series
     int pad[2];
 };
 

@@ -183,14 +183,13 @@ __global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ n
         gm.kappa = kap;
         if (!(rmax < 0.9) || !(kap < 0.5) || !isfinite(det) || det == 0.0) {
             gm.curv = -1.0;   // no usable bound: the splat scans whole images
-            gm.phased = 0;
         } else {
             gm.curv = kap;
-            // Two sources 3 apart map at least 3 (smin - kappa smax) apart; at
-            // >= 2 sqrt(2) their bilinear footprints are disjoint, so the nine
-            // source classes (i mod 3, j mod 3) can be splatted phase by phase.
-            gm.phased = (3.0 * (smin - kap * smax) >= 2.8284271247461903 + 0.02) ? 1 : 0;
         }
+        (void)smin;
+        // n' = 1 - sqrt(1 - r^2) = r^2 (1/2 + r^2/8 + r^4/16 + 5 r^6/128 + ...);
+        // for r^2 <= 0.01 the terms after r^8 are below 3e-12 of n'.
+        gm.series = (rmax * rmax <= 0.01) ? 1 : 0;
         gm.pad[0] = gm.pad[1] = 0;
         geom[i] = gm;
     }
@@ -302,30 +301,42 @@ __global__ __launch_bounds__(256) void k_bin(InputView in, VisRec* __restrict__ 
 
 __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ hist, int* __restrict__ bstart, int G)
 {
-    __shared__ int sh[1024];
+    // Exclusive prefix of the bucket counts: each thread sums a contiguous
+    // chunk, waves scan their chunk sums with shuffles, one wave scans the
+    // 16 wave totals.
+    __shared__ int wsum[16];
     const int s = blockIdx.x;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int per = (G + 1023) / 1024;
     const int beg = min(t * per, G), end = min(beg + per, G);
     const int* h = hist + (size_t)s * G;
     int* o = bstart + (size_t)s * (G + 1);
     int local = 0;
     for (int i = beg; i < end; ++i) local += h[i];
-    sh[t] = local;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partial sums.
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int v = (t >= d) ? sh[t - d] : 0;
-        __syncthreads();
-        sh[t] += v;
-        __syncthreads();
+    int x = local;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
     }
-    int run = sh[t] - local;
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const int y = __shfl_up(w, d);
+            if (lane >= d) w += y;
+        }
+        if (lane < 16) wsum[lane] = w;   // inclusive totals
+    }
+    __syncthreads();
+    int run = x - local + (wid > 0 ? wsum[wid - 1] : 0);
     for (int i = beg; i < end; ++i) {
         o[i] = run;
         run += h[i];
     }
-    if (t == 1023) o[G] = sh[1023];
+    if (t == 1023) o[G] = wsum[15];
 }
 
 // --------------------------------------------------------------- scatter
@@ -336,11 +347,13 @@ __global__ __launch_bounds__(256) void k_scatter(const VisRec* __restrict__ rec,
 {
     const int s = blockIdx.y;
     const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= meta[s].nvis) return;
+    // n < max count <= stride, so the loads are in bounds before the count
+    // of this snapshot is known; all three are in flight together.
     const size_t q = (size_t)s * stride + (size_t)n;
     const int2 key = bs[q];
-    if (key.x < 0) return;
-    sorted[(size_t)s * stride + (size_t)(bstart[(size_t)s * (G + 1) + key.x] + key.y)] = rec[q];
+    const VisRec r = rec[q];
+    if (n >= meta[s].nvis || key.x < 0) return;
+    sorted[(size_t)s * stride + (size_t)(bstart[(size_t)s * (G + 1) + key.x] + key.y)] = r;
 }
 
 // ---------------------------------------------------------------- stage 1
@@ -719,7 +732,12 @@ __device__ __forceinline__ void sin_shift(const SnapGeom& g, double q1, double q
     const float l = (float)(-g.delta * (q1 - 1.0));
     const float mm = (float)(g.delta * (q2 - 1.0));
     const float r2 = l * l + mm * mm;
-    if (r2 <= 1.0f) {
+    if (g.series) {
+        // every source of the image has r^2 <= 0.01 (checked in k_reset)
+        const float np = r2 * fmaf(r2, fmaf(r2, fmaf(r2, 0.0390625f, 0.0625f), 0.125f), 0.5f);
+        o1 = q1 + g.sx * (double)np;
+        o2 = q2 + g.sy * (double)np;
+    } else if (r2 <= 1.0f) {
         // 1 - sqrt(1 - r^2) = r^2 / (1 + sqrt(1 - r^2)); hardware sqrt/rcp (1 ulp)
         const float np = r2 * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - r2));
         o1 = q1 + g.sx * (double)np;
