@@ -124,6 +124,25 @@ FIP_PLAN(6144, 256, 8, 8, 8, 4, 3);
 FIP_PLAN(12288, 512, 8, 8, 8, 8, 3);
 #undef FIP_PLAN
 
+// Padded LDS offsets of the R elements of a butterfly relative to the padded
+// index of its first element, when they are compile-time constants (so the
+// LDS instructions carry them as immediates):
+//  - loads j + r*NB with NB % 32 == 0: r * (NB + NB/32);
+//  - stores base + r*Ns, base = (j/Ns)*Ns*R + (j%Ns): if Ns*R divides 32,
+//    base % 32 + r*Ns < 32 and the offset is r*Ns; if Ns % 32 == 0, or Ns
+//    divides 32 and 32 divides Ns*R, base % 32 < Ns and the offset is
+//    r*Ns + (r*Ns)/32.
+template <int NB> struct LoadOff {
+    static constexpr bool ok = (NB % 32) == 0;
+    static constexpr __host__ __device__ int at(int r) { return r * (NB + NB / 32); }
+};
+template <int Ns, int R> struct StoreOff {
+    static constexpr bool small = (32 % (Ns * R)) == 0;
+    static constexpr bool big = (Ns % 32 == 0) || ((32 % Ns) == 0 && ((Ns * R) % 32) == 0);
+    static constexpr bool ok = small || big;
+    static constexpr __host__ __device__ int at(int r) { return small ? r * Ns : r * Ns + ((r * Ns) >> 5); }
+};
+
 // One Stockham stage of radix R with Ns = product of earlier radices.
 // tw points at this stage's twiddles laid out [r-1][k], k < Ns.
 template <int N, int NT, int R, int Ns>
@@ -149,8 +168,14 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ buf, const float2
     for (int b = 0; b < BPT; ++b) {
         const int j = t + b * NT;
         if (exact || j < NB) {
+            if constexpr (LoadOff<NB>::ok) {
+                const float2* p = buf + lds_pad(j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = buf[lds_pad(j + r * NB)];
+                for (int r = 0; r < R; ++r) v[b][r] = p[LoadOff<NB>::at(r)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b][r] = buf[lds_pad(j + r * NB)];
+            }
         }
     }
     __syncthreads();
@@ -165,8 +190,14 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ buf, const float2
             }
             Bfly<R>::run(v[b]);
             const int base = (j / Ns) * Ns * R + k;
+            if constexpr (StoreOff<Ns, R>::ok) {
+                float2* p = buf + lds_pad(base);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * Ns)] = v[b][r];
+                for (int r = 0; r < R; ++r) p[StoreOff<Ns, R>::at(r)] = v[b][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[lds_pad(base + r * Ns)] = v[b][r];
+            }
         }
     }
     __syncthreads();
