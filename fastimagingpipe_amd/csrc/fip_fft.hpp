@@ -101,7 +101,9 @@ template <> struct Bfly<8> {
 
 // ----------------------------------------------------------------- FFT plans
 // Radix lists for N = 3 * 2^k.  The host twiddle generator walks the same
-// list (fip_plan.cpp), so the two stay consistent by construction.
+// list (fip_plan.cpp), so the two stay consistent by construction.  From
+// N = 1536 on, a transform gets N/12 threads (12 points per thread): more
+// waves per workgroup hide the LDS and barrier latency of each stage.
 template <int NT, int... Rs>
 struct PlanDef {
     static constexpr int nt = NT;
@@ -118,10 +120,10 @@ FIP_PLAN(96, 64, 8, 4, 3);
 FIP_PLAN(192, 64, 8, 8, 3);
 FIP_PLAN(384, 64, 8, 8, 2, 3);
 FIP_PLAN(768, 64, 8, 8, 4, 3);
-FIP_PLAN(1536, 64, 8, 8, 8, 3);
-FIP_PLAN(3072, 128, 8, 8, 8, 2, 3);
-FIP_PLAN(6144, 256, 8, 8, 8, 4, 3);
-FIP_PLAN(12288, 512, 8, 8, 8, 8, 3);
+FIP_PLAN(1536, 128, 8, 8, 8, 3);
+FIP_PLAN(3072, 256, 8, 8, 8, 2, 3);
+FIP_PLAN(6144, 512, 8, 8, 8, 4, 3);
+FIP_PLAN(12288, 1024, 8, 8, 8, 8, 3);
 #undef FIP_PLAN
 
 // Padded LDS offsets of the R elements of a butterfly relative to the padded

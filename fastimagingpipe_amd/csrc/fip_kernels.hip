@@ -557,7 +557,7 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
 }
 
 template <int G, int KR>
-__global__ __launch_bounds__(FftPlan<G>::nt, 2) void k_stage1(const VisRec* __restrict__ sorted,
+__global__ __launch_bounds__(FftPlan<G>::nt, (FftPlan<G>::nt >= 1024 ? 1 : 2)) void k_stage1(const VisRec* __restrict__ sorted,
                                                             const int* __restrict__ bstart,
                                                             const SnapMeta* __restrict__ meta,
                                                             const float2* __restrict__ tw,
