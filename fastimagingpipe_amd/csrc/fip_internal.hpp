@@ -50,7 +50,8 @@ struct SnapGeom {
     double kappa;                // bound on |grad| of the SIN shift over the image
     int sin_on;                  // xi^2 + eta^2 != 0
     int series;                  // every direction cosine has r^2 <= 0.01: n' by its power series
-    int pad[2];
+    int ccap;                    // upper bound on the sources whose splat reaches one pixel
+    int pad;
 };
 
 enum VisType { kVisF32 = 0, kVisBF16 = 1, kVisF16 = 2 };

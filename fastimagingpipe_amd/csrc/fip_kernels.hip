@@ -186,11 +186,24 @@ __global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ n
         } else {
             gm.curv = kap;
         }
-        (void)smin;
+        // A pixel receives splat weight from the sources mapped into the open
+        // 2 x 2 square around it.  The map expands by at least
+        // smin (1 - kappa), so they lie in a set of diameter
+        // 2 sqrt(2) / (smin (1 - kappa)), which holds at most (d + 1)^2
+        // integer points.  Without a usable bound every pixel of the image
+        // counts.
+        {
+            const double den = smin * (1.0 - kap);
+            double cap = (gm.curv >= 0.0 && den > 0.0) ? 2.8284271247461903 / den + 1.0 : 0.0;
+            cap = cap * cap;
+            const double all = (double)I * (double)I;
+            gm.ccap = (int)fmin(gm.curv >= 0.0 && den > 0.0 ? ceil(cap) : all, fmin(all, 1073741824.0));
+            gm.ccap = max(gm.ccap, 1);
+        }
         // n' = 1 - sqrt(1 - r^2) = r^2 (1/2 + r^2/8 + r^4/16 + 5 r^6/128 + ...);
         // for r^2 <= 0.01 the terms after r^8 are below 3e-12 of n'.
         gm.series = (rmax * rmax <= 0.01) ? 1 : 0;
-        gm.pad[0] = gm.pad[1] = 0;
+        gm.pad = 0;
         geom[i] = gm;
     }
 }
@@ -753,12 +766,13 @@ __device__ __forceinline__ void source_position(const SnapGeom& g, int i, int j,
     sin_shift(g, g.m00 * a + g.m01 * b, g.m10 * a + g.m11 * b, LL, MM);
 }
 
-// Exact-integer accumulation: contributions are rounded to integers (scaled
-// by a power of two) and summed as doubles.  Every partial sum stays below
-// 2^53, so each sum is exact and independent of the order of the LDS atomics.
-__device__ __forceinline__ void acc_exact(double* a, double x)
+// Exact integer accumulation: contributions are scaled by a power of two,
+// rounded to integers and summed as int32; the scale keeps every pixel sum
+// below 2^29 in magnitude, so each sum is exact and independent of the
+// order of the LDS atomics.
+__device__ __forceinline__ void acc_exact(int* a, float x)
 {
-    atomicAdd(a, rint(x));
+    atomicAdd(a, (int)rintf(x));
 }
 
 // Per target tile: bounding box and per-row ranges of the source pixels whose
@@ -781,7 +795,8 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     constexpr int A = TS + 2;             // apron: targets -1 .. TS
     constexpr int UB = 8;                 // sources in flight per lane
     constexpr int CHK = 512;              // runs of 64 numbered sources per window
-    __shared__ __align__(16) double acc[A * A];
+    static_assert((A * A) % 4 == 0, "16-byte clears");
+    __shared__ __align__(16) int acc[A * A];
     __shared__ int rlo[ROW_CAP], rcnt[ROW_CAP], rofs[ROW_CAP + 1];
     __shared__ unsigned char crow[CHK];   // row in which each run of 64 starts
     __shared__ double qb[6];
@@ -853,19 +868,19 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
         bb[3] = j1;
     }
     {
-        // 16-byte stores (A * A is even and acc is 16-byte aligned).
-        double2* a2 = reinterpret_cast<double2*>(acc);
-        for (int q = tid; q < A * A / 2; q += NT) a2[q] = make_double2(0.0, 0.0);
+        int4* a4 = reinterpret_cast<int4*>(acc);
+        for (int q = tid; q < A * A / 4; q += NT) a4[q] = make_int4(0, 0, 0, 0);
     }
     __syncthreads();
     const int i0 = bb[0], i1 = bb[1], j0 = bb[2], j1 = bb[3];
     const int H = j1 - j0 + 1;
-    // 2^k * max contribution <= 2^44: sums of up to 256 contributions stay
-    // exact integers below 2^52.
+    // A pixel sums at most ccap contributions of at most max(pre)/N each
+    // (1/N of FIPkernels.cu:391); 2^k * ccap * max(pre)/N < 2^29.
     int e = 0;
-    frexpf(__uint_as_float(m.maxp_bits) / (float)m.norm_n, &e);
-    const double vscale = ldexp(1.0, 44 - e) / (double)m.norm_n;   // 1/N of FIPkernels.cu:391
-    const double iscale = ldexp(1.0, e - 44);
+    frexpf(fmaxf((float)g.ccap * (__uint_as_float(m.maxp_bits) / (float)m.norm_n), 1e-30f), &e);
+    const int ks = min(29 - e, 120);
+    const float vscale = ldexpf(1.0f, ks) / (float)m.norm_n;
+    const float iscale = ldexpf(1.0f, -ks);
     const double lim = (double)(h - 1);
     const float* ps = pre + (size_t)s * I * I;
     const double ql1 = qb[0], qh1 = qb[1], ql2 = qb[2], qh2 = qb[3];
@@ -961,15 +976,15 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
                         const double fl = floor(LL), fm = floor(MM);
                         const int ra = (int)fm + h - R0, ca = (int)fl + h - C0;
                         if ((unsigned)(ra + 1) < (unsigned)(TS + 1) && (unsigned)(ca + 1) < (unsigned)(TS + 1)) {
-                            const double wl1 = LL - fl, wm1 = MM - fm;
-                            const double vs = (double)v * vscale;
-                            const double t0 = (1.0 - wl1) * vs, t1 = wl1 * vs;
+                            const float wl1 = (float)(LL - fl), wm1 = (float)(MM - fm);
+                            const float vs = v * vscale;
+                            const float t0 = (1.0f - wl1) * vs, t1 = wl1 * vs;
                             // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL); a
                             // zero weight on an integer position adds an exact zero.
-                            double* p0 = &acc[(ra + 1) * A + (ca + 1)];
-                            acc_exact(p0, t0 * (1.0 - wm1));
+                            int* p0 = &acc[(ra + 1) * A + (ca + 1)];
+                            acc_exact(p0, t0 * (1.0f - wm1));
                             acc_exact(p0 + A, t0 * wm1);
-                            acc_exact(p0 + 1, t1 * (1.0 - wm1));
+                            acc_exact(p0 + 1, t1 * (1.0f - wm1));
                             acc_exact(p0 + A + 1, t1 * wm1);
                         }
                     }
@@ -984,9 +999,9 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     // Four consecutive pixels per thread per pass, one 16-byte store.
     for (int q = tid; q < TS * TS / 4; q += NT) {
         const int rr = q / (TS / 4), cc = 4 * (q % (TS / 4));
-        const double* a = acc + (rr + 1) * A + cc + 1;
-        const float4 v = make_float4((float)(a[0] * iscale), (float)(a[1] * iscale), (float)(a[2] * iscale),
-                                     (float)(a[3] * iscale));
+        const int* a = acc + (rr + 1) * A + cc + 1;
+        const float4 v = make_float4((float)a[0] * iscale, (float)a[1] * iscale, (float)a[2] * iscale,
+                                     (float)a[3] * iscale);
         *reinterpret_cast<float4*>(out + (size_t)(R0 + rr) * I + C0 + cc) = v;
         lmax = fmaxf(lmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
     }
