@@ -782,10 +782,11 @@ __device__ __forceinline__ void acc_exact(int* a, float x)
 // row by row; lane work is a contiguous range of that numbering, so every
 // lane of every wave has a source whatever the rotation of the map.  A
 // lane finds its row from a table of the rows in which each run of 64
-// numbers starts (usually no further step) and keeps eight independent
-// loads and position chains in flight.
+// numbers starts (usually no further step) and keeps four independent
+// loads and position chains in flight; at most 64 VGPRs keep eight waves
+// per SIMD resident.
 template <int TS>
-__global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, SnapMeta* __restrict__ meta,
+__global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre, SnapMeta* __restrict__ meta,
                                                const SnapGeom* __restrict__ geom, float* __restrict__ img,
                                                int I)
 {
@@ -793,7 +794,7 @@ __global__ __launch_bounds__(256) void k_splat(const float* __restrict__ pre, Sn
     constexpr int NW = NT / 64;           // waves
     constexpr int ROW_CAP = 2 * TS + 32;   // source rows handled per pass
     constexpr int A = TS + 2;             // apron: targets -1 .. TS
-    constexpr int UB = 8;                 // sources in flight per lane
+    constexpr int UB = 4;                 // sources in flight per lane
     constexpr int CHK = 512;              // runs of 64 numbered sources per window
     static_assert((A * A) % 4 == 0, "16-byte clears");
     __shared__ __align__(16) int acc[A * A];
