@@ -12,9 +12,11 @@ namespace fip {
 // ES gridding kernel shape parameter and support (FIPkernels.cu:198, 205).
 constexpr double kBeta = 15.3704324328;
 constexpr int kSupport = 8;
-// K tile height: stage 1 writes 16 consecutive image rows of one uv column as
-// one 128-byte line, stage 2 reads groups of rows from the same line.
-constexpr int kKTile = 16;
+// K tile height: stage 1 writes 8 consecutive image rows of one uv column as
+// one 64-byte run, stage 2 reads the rows of its workgroup from each run
+// (measured best among 2, 4, 8 and 16 rows; C2R rows per workgroup must
+// divide it).
+constexpr int kKTile = 8;
 
 // One binned visibility: integer floor and FP32 fraction of each grid
 // coordinate (computed from an FP64 product B * cell * g), and the weighted

@@ -1195,13 +1195,13 @@ static void stage2_pick(const WorkView& w, const PlanTables& p, int S, const Tun
 {
     // LDS per row is (M + M/32 + 2) * 8 bytes; keep a workgroup under 128 KiB.
     constexpr int row_bytes = lds_len(M + 1) * 8;
-    if constexpr (4 * row_bytes <= 128 * 1024) {
+    if constexpr (4 * row_bytes <= 128 * 1024 && kKTile % 4 == 0) {
         if (tu.stage2_rows >= 4) {
             stage2_dispatch<M, 4>(w, p, S, st);
             return;
         }
     }
-    if constexpr (2 * row_bytes <= 128 * 1024) {
+    if constexpr (2 * row_bytes <= 128 * 1024 && kKTile % 2 == 0) {
         if (tu.stage2_rows >= 2) {
             stage2_dispatch<M, 2>(w, p, S, st);
             return;
