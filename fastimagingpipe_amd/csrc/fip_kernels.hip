@@ -645,59 +645,48 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
     float2* z = zb + grp * L;
     const float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
 
-    // Gather K[y][c], c = 0..cmax, for the ROWS rows; zero the rest.  The
-    // loads of GB consecutive passes are issued before any LDS store, so one
-    // HBM/L2 latency is paid per GB passes instead of per pass.
-    for (int c = cmax + 1 + t; c <= M; c += NT) z[lds_pad(c)] = make_float2(0.0f, 0.0f);
+    // Gather and C2R split in one pass: Z[k] = E[k] + i O[k],
+    // E = K[k] + K~[k+M], O = (K[k] - K~[k+M]) e^{2 pi i k/G},
+    // K~[k+M] = conj(K[M-k]) (k > 0), K[M] (k = 0).  A thread takes the pairs
+    // (p, M-p) of every row of the workgroup, reads K[p] and K[M-p] straight
+    // from the K tiles (zero beyond cmax) and writes Z[p], Z[M-p]; the pairs
+    // cover every index 0..M-1.  wpost[M - p] = -conj(wpost[p]).
     {
-        constexpr int GB = 8;
-        const int nel = (cmax + 1) * ROWS;
+        constexpr int PB = 4;   // pairs per thread per pass, loads issued together
         const float2* kt0 = ks + (size_t)(y0 / kKTile) * (M + 1) * kKTile + (y0 % kKTile);
-        for (int base = threadIdx.x; base < nel; base += NT * ROWS * GB) {
-            float2 v[GB];
+        for (int p0 = threadIdx.x; p0 <= M / 2; p0 += NT * ROWS * PB) {
+            float2 a[PB][ROWS], b[PB][ROWS], wv[PB];
 #pragma unroll
-            for (int u = 0; u < GB; ++u) {
-                const int L2 = base + u * NT * ROWS;
-                const int c = L2 / ROWS, r = L2 % ROWS;
-                if (L2 < nel) v[u] = kt0[(size_t)c * kKTile + r];
-            }
+            for (int u = 0; u < PB; ++u) {
+                const int p = p0 + u * NT * ROWS;
+                const bool ok = p <= M / 2;
+                wv[u] = ok ? wpost[p] : make_float2(0.0f, 0.0f);
 #pragma unroll
-            for (int u = 0; u < GB; ++u) {
-                const int L2 = base + u * NT * ROWS;
-                if (L2 < nel) zb[(L2 % ROWS) * L + lds_pad(L2 / ROWS)] = v[u];
-            }
-        }
-    }
-    __syncthreads();
-    // C2R split: Z[k] = E[k] + i O[k], E = K[k] + K~[k+M], O = (K[k] - K~[k+M]) e^{2 pi i k/G},
-    // K~[k+M] = conj(K[M-k]) (k > 0), K[M] (k = 0).  Pairs (p, M-p) are disjoint.
-    // wpost[M - p] = -conj(wpost[p]) (e^{2 pi i (M-p)/G} with G = 2M); the
-    // loads of four pairs per thread are issued together.
-    for (int p0 = t; p0 <= M / 2; p0 += 4 * NT) {
-        float2 wv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int p = p0 + u * NT;
-            wv[u] = (p <= M / 2) ? wpost[p] : make_float2(0.0f, 0.0f);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int p = p0 + u * NT;
-            if (p > M / 2) break;
-            const float2 a = z[lds_pad(p)];
-            const float2 b = z[lds_pad(M - p)];
-            if (p == 0) {
-                z[lds_pad(0)] = cadd(cadd(a, b), cmul_i(csub(a, b)));
-            } else {
-                const float2 kp = cconj(b);   // K~[p+M]
-                const float2 zp = cadd(cadd(a, kp), cmul_i(cmul(csub(a, kp), wv[u])));
-                if (p != M - p) {
-                    const float2 km = cconj(a);   // K~[M-p+M]
-                    const float2 wm = make_float2(-wv[u].x, wv[u].y);
-                    const float2 zm = cadd(cadd(b, km), cmul_i(cmul(csub(b, km), wm)));
-                    z[lds_pad(M - p)] = zm;
+                for (int r = 0; r < ROWS; ++r) {
+                    a[u][r] = (ok && p <= cmax) ? kt0[(size_t)p * kKTile + r] : make_float2(0.0f, 0.0f);
+                    b[u][r] = (ok && M - p <= cmax) ? kt0[(size_t)(M - p) * kKTile + r] : make_float2(0.0f, 0.0f);
                 }
-                z[lds_pad(p)] = zp;
+            }
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int p = p0 + u * NT * ROWS;
+                if (p > M / 2) break;
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r) {
+                    float2* zr = zb + r * L;
+                    const float2 av = a[u][r], bv = b[u][r];
+                    if (p == 0) {
+                        zr[lds_pad(0)] = cadd(cadd(av, bv), cmul_i(csub(av, bv)));
+                    } else {
+                        const float2 kp = cconj(bv);   // K~[p+M]
+                        zr[lds_pad(p)] = cadd(cadd(av, kp), cmul_i(cmul(csub(av, kp), wv[u])));
+                        if (p != M - p) {
+                            const float2 km = cconj(av);   // K~[M-p+M]
+                            const float2 wm = make_float2(-wv[u].x, wv[u].y);
+                            zr[lds_pad(M - p)] = cadd(cadd(bv, km), cmul_i(cmul(csub(bv, km), wm)));
+                        }
+                    }
+                }
             }
         }
     }
