@@ -554,32 +554,37 @@ int fip_autotune(size_t image_size, size_t num_baselines, float cell_size, int r
         std::lock_guard<std::recursive_mutex> lk(C->mu);
         Plan* P = nullptr;
         if (int rc = C->plan(image_size, &P)) return rc;
-        // Synthetic snapshot: baselines filling 55% of the grid width with a
-        // centrally concentrated density, random visibilities, tilted plane.
+        // Synthetic workload of the caller's shape from the library's own
+        // generator: three snapshots of a log-radial array whose longest
+        // baseline fills about 54% of the half grid at this cell size
+        // (2 R f / c * cell = 0.27), as the plain path images them.
         const size_t nbl = num_baselines;
-        std::mt19937_64 rng(12345);
-        std::uniform_real_distribution<double> U(0.0, 1.0);
-        std::vector<float> vr(nbl), vi(nbl), b(2 * nbl);
-        const double rmax = 0.27 / (double)cell_size;
-        for (size_t n = 0; n < nbl; ++n) {
-            const double r = rmax * std::pow(U(rng), 1.5), a = 2.0 * M_PI * U(rng);
-            b[2 * n] = (float)(r * std::cos(a));
-            b[2 * n + 1] = (float)(r * std::sin(a));
-            vr[n] = (float)(U(rng) - 0.5);
-            vi[n] = (float)(U(rng) - 0.5);
+        int nant = 2;
+        while ((size_t)nant * (size_t)(nant - 1) / 2 < nbl) ++nant;
+        const size_t nall = (size_t)nant * (size_t)(nant - 1) / 2;
+        constexpr int S = 3;
+        const double freq = 140e6, c_light = 299792458.0;
+        const double radius = 0.135 * c_light / (freq * (double)cell_size);
+        std::vector<float> vr(S * nall), vi(S * nall), b(2 * S * nall), v(9 * S);
+        if (int rc = fip_synth(nant, radius, S, 1.0, freq, -30.0, -26.7, image_size, cell_size, 20, 0.01, 1.0, -1.8,
+                               0.1, 0, nullptr, 0.0, 0.0, 12345ull, vr.data(), vi.data(), b.data(), v.data(), nullptr))
+            return rc;
+        // First nbl rows of every snapshot, packed at stride nbl.
+        std::vector<float> pr(S * nbl), pi(S * nbl), pb(2 * S * nbl);
+        for (int s = 0; s < S; ++s) {
+            std::copy(vr.begin() + s * nall, vr.begin() + s * nall + nbl, pr.begin() + s * nbl);
+            std::copy(vi.begin() + s * nall, vi.begin() + s * nall + nbl, pi.begin() + s * nbl);
+            std::copy(b.begin() + 2 * s * nall, b.begin() + 2 * (s * nall + nbl), pb.begin() + 2 * s * nbl);
         }
-        const double tilt = 0.3;
-        const float v[9] = {(float)std::cos(tilt), 0.0f, (float)std::sin(tilt), 0.0f, 1.0f, 0.0f,
-                            (float)(-std::sin(tilt)), 0.0f, (float)std::cos(tilt)};
-        FIP_HIP(C->in_vr.ensure(nbl * 4));
-        FIP_HIP(C->in_vi.ensure(nbl * 4));
-        FIP_HIP(C->in_b.ensure(nbl * 8));
-        FIP_HIP(C->in_v.ensure(36));
-        FIP_HIP(hipMemcpy(C->in_vr.p, vr.data(), nbl * 4, hipMemcpyHostToDevice));
-        FIP_HIP(hipMemcpy(C->in_vi.p, vi.data(), nbl * 4, hipMemcpyHostToDevice));
-        FIP_HIP(hipMemcpy(C->in_b.p, b.data(), nbl * 8, hipMemcpyHostToDevice));
-        FIP_HIP(hipMemcpy(C->in_v.p, v, 36, hipMemcpyHostToDevice));
-        if (int rc = C->work.ensure(1, nbl, P->t)) return rc;
+        FIP_HIP(C->in_vr.ensure(S * nbl * 4));
+        FIP_HIP(C->in_vi.ensure(S * nbl * 4));
+        FIP_HIP(C->in_b.ensure(S * nbl * 8));
+        FIP_HIP(C->in_v.ensure(S * 36));
+        FIP_HIP(hipMemcpy(C->in_vr.p, pr.data(), S * nbl * 4, hipMemcpyHostToDevice));
+        FIP_HIP(hipMemcpy(C->in_vi.p, pi.data(), S * nbl * 4, hipMemcpyHostToDevice));
+        FIP_HIP(hipMemcpy(C->in_b.p, pb.data(), S * nbl * 8, hipMemcpyHostToDevice));
+        FIP_HIP(hipMemcpy(C->in_v.p, v.data(), S * 36, hipMemcpyHostToDevice));
+        if (int rc = C->work.ensure(S, nbl, P->t)) return rc;
         InputView in{C->in_vr.p, C->in_vi.p, C->in_b.as<float>(), C->in_v.as<float>(), nullptr, nbl, (int)nbl,
                      FIP_VIS_F32};
         const bool was = C->profiling;
@@ -594,7 +599,7 @@ int fip_autotune(size_t image_size, size_t num_baselines, float cell_size, int r
                 C->stage_n[i] = 0;
             }
             for (int r = 0; r < reps + 1; ++r) {   // run 0 is a warm-up
-                if (int rc = run_imaging(*C, *P, C->work, in, 1, (int)nbl, cell_size, nullptr, (int)nbl, t,
+                if (int rc = run_imaging(*C, *P, C->work, in, S, (int)nbl, cell_size, nullptr, (int)nbl, t,
                                          C->stream))
                     return rc;
                 FIP_HIP(hipStreamSynchronize(C->stream));
@@ -611,11 +616,11 @@ int fip_autotune(size_t image_size, size_t num_baselines, float cell_size, int r
         };
         Tuning best = C->tuning(image_size);
         double best_s1 = 1e300, best_s2 = 1e300, best_sp = 1e300;
-        const int records[2] = {4, 8};
+        const int records[3] = {2, 4, 8};
         const int rows[3] = {1, 2, 4};
         const int tiles[2] = {32, 64};
         double tm[kStCount];
-        for (int ci = 0; ci < 2; ++ci) {
+        for (int ci = 0; ci < 3; ++ci) {
             for (int pers = 0; pers < 2; ++pers) {
                 Tuning t = best;
                 t.stage1_records = records[ci];
