@@ -1001,15 +1001,17 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
 
 // --------------------------------------------------------------- trigger
 
+// One wave per tile (four tiles per workgroup): per-lane partial sums over
+// 16-byte loads, then wave reductions only (FIPkernels.cu:499-552).
 __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, const SnapMeta* __restrict__ meta,
                                                  const int* __restrict__ win, int I, int T,
                                                  float* __restrict__ out)
 {
-    constexpr int NT = 256;
-    __shared__ float sh[3][NT / 64];
     const int w = blockIdx.y;
     const int un = I / T;
-    const int tile = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= un * un) return;   // whole wave
     const int ti = tile / un, tj = tile % un;
     const int s1 = win[3 * w], s2 = win[3 * w + 1], s3 = win[3 * w + 2];
     const float* d1 = img + (size_t)s1 * I * I;
@@ -1026,20 +1028,33 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
         sr += (ratio < 1.0f) ? ratio : 1.0f;
     };
     if ((T & 3) == 0) {
-        // 16-byte loads: rows of a tile start at multiples of 4 floats.
-        const int T4 = T >> 2;
-        for (int p = threadIdx.x; p < T * T4; p += NT) {
-            const size_t q = (size_t)(ti * T + p / T4) * I + (size_t)(tj * T + 4 * (p % T4));
-            const float4 a = *reinterpret_cast<const float4*>(d1 + q);
-            const float4 b = *reinterpret_cast<const float4*>(d2 + q);
-            const float4 c = *reinterpret_cast<const float4*>(d3 + q);
-            px(a.x, b.x, c.x);
-            px(a.y, b.y, c.y);
-            px(a.z, b.z, c.z);
-            px(a.w, b.w, c.w);
+        // 16-byte loads: rows of a tile start at multiples of 4 floats; the
+        // loads of four passes are issued together.
+        const int T4 = T >> 2, n4 = T * T4;
+        for (int p0 = lane; p0 < n4; p0 += 4 * 64) {
+            float4 a[4], b[4], c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int p = p0 + u * 64;
+                if (p < n4) {
+                    const size_t q = (size_t)(ti * T + p / T4) * I + (size_t)(tj * T + 4 * (p % T4));
+                    a[u] = *reinterpret_cast<const float4*>(d1 + q);
+                    b[u] = *reinterpret_cast<const float4*>(d2 + q);
+                    c[u] = *reinterpret_cast<const float4*>(d3 + q);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (p0 + u * 64 < n4) {
+                    px(a[u].x, b[u].x, c[u].x);
+                    px(a[u].y, b[u].y, c[u].y);
+                    px(a[u].z, b[u].z, c[u].z);
+                    px(a[u].w, b[u].w, c[u].w);
+                }
+            }
         }
     } else {
-        for (int p = threadIdx.x; p < T * T; p += NT) {
+        for (int p = lane; p < T * T; p += 64) {
             const size_t q = (size_t)(ti * T + p / T) * I + (size_t)(tj * T + p % T);
             px(d1[q], d2[q], d3[q]);
         }
@@ -1047,25 +1062,12 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
     sum = wave_sum_f(sum);
     mx = wave_max_f(mx);
     sr = wave_sum_f(sr);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
-        sh[0][wid] = sum;
-        sh[1][wid] = mx;
-        sh[2][wid] = sr;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float S = 0.0f, MX = 0.0f, SR = 0.0f;
-        for (int k = 0; k < NT / 64; ++k) {
-            S += sh[0][k];
-            MX = fmaxf(MX, sh[1][k]);
-            SR += sh[2][k];
-        }
         const float m3 = fmaxf(__uint_as_float(meta[s1].maxd_bits),
                                fmaxf(__uint_as_float(meta[s2].maxd_bits), __uint_as_float(meta[s3].maxd_bits)));
         const float Tf = (float)T;
         // FIPkernels.cu:550
-        out[(size_t)w * un * un + tile] = 1.0f - (S / Tf / Tf) * MX * (SR / Tf / Tf) / m3 / m3;
+        out[(size_t)w * un * un + tile] = 1.0f - (sum / Tf / Tf) * mx * (sr / Tf / Tf) / m3 / m3;
     }
 }
 
@@ -1232,7 +1234,8 @@ hipError_t launch_trigger(const WorkView& w, const PlanTables& p, const int* win
 {
     const int un = p.I / tile;
     if (un <= 0 || nwin <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_trigger, dim3(un * un, nwin), dim3(256), 0, st, w.img, w.meta, win, p.I, tile, out);
+    hipLaunchKernelGGL(k_trigger, dim3((un * un + 3) / 4, nwin), dim3(256), 0, st, w.img, w.meta, win, p.I, tile,
+                       out);
     return hipGetLastError();
 }
 
