@@ -396,3 +396,40 @@ def test_launch_tunings_give_identical_results(fip):
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
     finally:
         fip.set_tuning(w["image"], 4, 2, 64, 0)
+
+
+@pytest.mark.parametrize("tile", [1, 10, 24, 256])
+def test_tile_sizes_including_ragged(fip, oracle, tile):
+    """unit_size that does not divide the image (tiles cover the top-left
+    (I/T)*T square), odd sizes (scalar loads), one tile, one pixel per tile."""
+    w = SMALL
+    d = synth_workload(fip, w, seed=97)
+    got = fip.fipipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nbl"], w["image"], 3, w["cell"], tile)
+    ref = oracle.pipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nbl"], w["image"], 3, w["cell"], tile)
+    assert got.shape == (w["image"] // tile,) * 2
+    assert np.abs(got - ref).max() < 2e-5
+
+
+def test_extra_snapshots_are_ignored(fip):
+    """FIpipe images snapshots 0..2 only (FIPkernels.cu:647)."""
+    w = SMALL
+    d = synth_workload(fip, w, nsnap=5, seed=98)
+    nbl = d["nbl"]
+    five = fip.fipipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nbl, w["image"], 5, w["cell"], w["tile"])
+    three = fip.fipipe(d["vis_re"][:3], d["vis_im"][:3], d["bin"][:3], d["vin"][:3], nbl, w["image"], 3,
+                       w["cell"], w["tile"])
+    assert np.array_equal(five.view(np.uint32), three.view(np.uint32))
+
+
+def test_out_of_grid_baselines_only_count_in_the_normalisation(fip, oracle):
+    """Baselines whose taps fall outside the grid add nothing but still count in
+    the splat's 1/N, as in the reference."""
+    w = SMALL
+    d = synth_workload(fip, w, seed=99)
+    nbl = d["nbl"]
+    B = d["bin"].copy()
+    B[:, :40, 0] = 1e7          # far outside the uv grid
+    B[:, 40:60, 1] = -1e9
+    got = fip.fipipe(d["vis_re"], d["vis_im"], B, d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
+    ref = oracle.pipe(d["vis_re"], d["vis_im"], B, d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
+    assert np.abs(got - ref).max() < 2e-5
