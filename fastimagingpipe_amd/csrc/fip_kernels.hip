@@ -747,21 +747,41 @@ __device__ __forceinline__ void sin_shift(const SnapGeom& g, double q1, double q
     }
 }
 
-// Forward map of source pixel (i, j): affine part in FP64 (exact for the
-// integer offsets), SIN shift from the closed form (FIPkernels.cu:289-376).
-__device__ __forceinline__ void source_position(const SnapGeom& g, int i, int j, int h, double& LL, double& MM)
+// a * b + c for a, b < 2^24 in one full-rate instruction (the compiler
+// otherwise forms a quarter-rate 32-bit multiply).
+__device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c)
 {
-    const double a = (double)(i - h), b = (double)(j - h);
-    sin_shift(g, g.m00 * a + g.m01 * b, g.m10 * a + g.m11 * b, LL, MM);
+    unsigned r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
 }
 
-// Exact integer accumulation: contributions are scaled by a power of two,
-// rounded to integers and summed as int32; the scale keeps every pixel sum
-// below 2^29 in magnitude, so each sum is exact and independent of the
-// order of the LDS atomics.
+// FP32 copy of the geometry the splat needs per source.
+struct GeomF {
+    float m00, m10;   // affine step per source column
+    float sx, sy, delta;
+    int sin_on, series;
+};
+__device__ __forceinline__ GeomF geom_f(const SnapGeom& g)
+{
+    GeomF f;
+    f.m00 = (float)g.m00;
+    f.m10 = (float)g.m10;
+    f.sx = (float)g.sx;
+    f.sy = (float)g.sy;
+    f.delta = (float)g.delta;
+    f.sin_on = g.sin_on;
+    f.series = g.series;
+    return f;
+}
+
+// Exact integer accumulation: contributions (non-negative) are scaled by a
+// power of two, rounded half up to integers and summed as int32; the scale
+// keeps every pixel sum below 2^29 (plus half a unit per contribution), so
+// each sum is exact and independent of the order of the LDS atomics.
 __device__ __forceinline__ void acc_exact(int* a, float x)
 {
-    atomicAdd(a, (int)rintf(x));
+    atomicAdd(a, (int)(x + 0.5f));
 }
 
 // Per target tile: bounding box and per-row ranges of the source pixels whose
@@ -783,8 +803,9 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
     constexpr int NW = NT / 64;           // waves
     constexpr int ROW_CAP = 2 * TS + 32;   // source rows handled per pass
     constexpr int A = TS + 2;             // apron: targets -1 .. TS
-    constexpr int UB = 4;                 // sources in flight per lane
-    constexpr int CHK = 512;              // runs of 64 numbered sources per window
+    constexpr int GN = 8;                 // consecutive sources of a row per lane
+    constexpr int CHK = 512;              // runs of 64 numbered groups per window
+    static_assert(GN % 4 == 0 && (GN & (GN - 1)) == 0, "groups of 16-byte loads");
     static_assert((A * A) % 4 == 0, "16-byte clears");
     __shared__ __align__(16) int acc[A * A];
     __shared__ int rlo[ROW_CAP], rcnt[ROW_CAP], rofs[ROW_CAP + 1];
@@ -863,6 +884,14 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
     }
     __syncthreads();
     const int i0 = bb[0], i1 = bb[1], j0 = bb[2], j1 = bb[3];
+    // Accepted target offsets of a footprint's top-left corner: the apron
+    // -1..TS-1 intersected with |LL| < h - 1, i.e. floor(LL) in [1-h, h-2]
+    // (floor(LL) >= 2-h when LL is an integer).
+    const int cxlo = max(-1, 1 - C0), cxn = min(TS - 1, I - 2 - C0) - cxlo;
+    const int rylo = max(-1, 1 - R0), ryn = min(TS - 1, I - 2 - R0) - rylo;
+    const bool cedge = 1 - C0 >= -1, redge = 1 - R0 >= -1;   // tile on the left / top image edge
+    // Without a curvature bound positions are not bounded a priori.
+    const bool unbounded = g.curv < 0.0;
     const int H = j1 - j0 + 1;
     // A pixel sums at most ccap contributions of at most max(pre)/N each
     // (1/N of FIPkernels.cu:391); 2^k * ccap * max(pre)/N < 2^29.
@@ -871,7 +900,7 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
     const int ks = min(29 - e, 120);
     const float vscale = ldexpf(1.0f, ks) / (float)m.norm_n;
     const float iscale = ldexpf(1.0f, -ks);
-    const double lim = (double)(h - 1);
+    const GeomF gf = geom_f(g);
     const float* ps = pre + (size_t)s * I * I;
     const double ql1 = qb[0], qh1 = qb[1], ql2 = qb[2], qh2 = qb[3];
     for (int rb = 0; rb < H; rb += ROW_CAP) {
@@ -907,8 +936,9 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
                 ia = max(i0, (int)floor(alo + h) - 1);
                 n = max(0, min(i1, (int)ceil(ahi + h) + 1) - ia + 1);
             }
-            rlo[r] = ia;
-            rcnt[r] = n;
+            // Groups of GN pixels aligned to i % GN == 0 that cover [ia, ia + n).
+            rlo[r] = ia / GN;
+            rcnt[r] = n > 0 ? ((ia + n - 1) / GN) - (ia / GN) + 1 : 0;
         }
         __syncthreads();
         // Exclusive prefix of the per-row counts (wave 0).
@@ -938,45 +968,85 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
                 for (int k = (a - f0 + 63) >> 6; (k << 6) < b - f0; ++k) crow[k] = (unsigned char)r;
             }
             __syncthreads();
-            for (int base = 0; base < nf; base += NT * UB) {
-                int ii[UB], jj[UB];
-                float pv[UB];
+            for (int base = 0; base < nf; base += NT) {
+                const int fl = base + tid;
+                if (fl >= nf) break;
+                const int f = f0 + fl;
+                int r = crow[fl >> 6];
+                while (rofs[r + 1] <= f) ++r;
+                const int gi = rlo[r] + (f - rofs[r]);
+                const int jj = j0 + rb + r;
+                // Aligned 16-byte loads (I % GN == 0, so a group never crosses
+                // the end of a row).
+                float pv[GN];
+                const float4* src = reinterpret_cast<const float4*>(ps + (size_t)jj * I + GN * gi);
 #pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    const int fl = base + u * NT + tid;
-                    ii[u] = -1;
-                    jj[u] = 0;
-                    if (fl < nf) {
-                        const int f = f0 + fl;
-                        int r = crow[fl >> 6];
-                        while (rofs[r + 1] <= f) ++r;
-                        ii[u] = rlo[r] + (f - rofs[r]);
-                        jj[u] = j0 + rb + r;
-                    }
+                for (int u = 0; u < GN / 4; ++u) {
+                    const float4 q = src[u];
+                    pv[4 * u] = q.x;
+                    pv[4 * u + 1] = q.y;
+                    pv[4 * u + 2] = q.z;
+                    pv[4 * u + 3] = q.w;
                 }
+                // Group origin in FP64: the affine position of its first pixel
+                // (clamped: positions beyond 1e6 px are rejected below), split
+                // into an integer part O and a fraction.  The pixels are placed
+                // in FP32 relative to O (local coordinates stay within a few
+                // pixels, ~1e-7 px), and only integer parts are recombined.
+                const double a0 = (double)(GN * gi - h), b = (double)(jj - h);
+                const double q1d = fmin(fmax(fma(g.m00, a0, g.m01 * b), -1e6), 1e6);
+                const double q2d = fmin(fmax(fma(g.m10, a0, g.m11 * b), -1e6), 1e6);
+                const double O1 = floor(q1d), O2 = floor(q2d);
+                const float r1 = (float)(q1d - O1), r2 = (float)(q2d - O2);
+                const int c1 = (int)O1 + h - C0, c2 = (int)O2 + h - R0;   // target offsets in the tile
+                // SIN direction cosines along the group: l = -cell (P1 - c0),
+                // m = cell (P2 - c0), c0 = I/2 + 1 (FIPkernels.cu:315-316); the
+                // origin minus one is an exact FP32 integer.
+                const float l0 = -gf.delta * ((float)(O1 - 1.0) + r1), dl = -gf.delta * gf.m00;
+                const float n0 = gf.delta * ((float)(O2 - 1.0) + r2), dn = gf.delta * gf.m10;
 #pragma unroll
-                for (int u = 0; u < UB; ++u) pv[u] = ii[u] >= 0 ? ps[(size_t)jj[u] * I + ii[u]] : 0.0f;
-#pragma unroll
-                for (int u = 0; u < UB; ++u) {
-                    const float v = pv[u];
-                    if (v == 0.0f) continue;
-                    double LL, MM;
-                    source_position(g, ii[u], jj[u], h, LL, MM);
-                    if (fabs(LL) < lim && fabs(MM) < lim) {
-                        const double fl = floor(LL), fm = floor(MM);
-                        const int ra = (int)fm + h - R0, ca = (int)fl + h - C0;
-                        if ((unsigned)(ra + 1) < (unsigned)(TS + 1) && (unsigned)(ca + 1) < (unsigned)(TS + 1)) {
-                            const float wl1 = (float)(LL - fl), wm1 = (float)(MM - fm);
-                            const float vs = v * vscale;
-                            const float t0 = (1.0f - wl1) * vs, t1 = wl1 * vs;
-                            // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL); a
-                            // zero weight on an integer position adds an exact zero.
-                            int* p0 = &acc[(ra + 1) * A + (ca + 1)];
-                            acc_exact(p0, t0 * (1.0f - wm1));
-                            acc_exact(p0 + A, t0 * wm1);
-                            acc_exact(p0 + 1, t1 * (1.0f - wm1));
-                            acc_exact(p0 + A + 1, t1 * wm1);
+                for (int k = 0; k < GN; ++k) {
+                    float o1 = fmaf((float)k, gf.m00, r1), o2 = fmaf((float)k, gf.m10, r2);
+                    if (gf.sin_on) {
+                        const float l = fmaf((float)k, dl, l0), mm = fmaf((float)k, dn, n0);
+                        const float rr = l * l + mm * mm;
+                        float np = 0.0f;
+                        if (gf.series) {
+                            // every source of the image has r^2 <= 0.01 (checked in k_reset)
+                            np = rr * fmaf(rr, fmaf(rr, fmaf(rr, 0.0390625f, 0.0625f), 0.125f), 0.5f);
+                        } else if (rr <= 1.0f) {
+                            // 1 - sqrt(1 - r^2) = r^2 / (1 + sqrt(1 - r^2)); hardware sqrt/rcp
+                            np = rr * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - rr));
                         }
+                        o1 = fmaf(gf.sx, np, o1);
+                        o2 = fmaf(gf.sy, np, o2);
+                    }
+                    bool fin = true;
+                    if (unbounded) {
+                        // finite and small enough for the integer recombination
+                        fin = fabsf(o1) < 1048576.0f && fabsf(o2) < 1048576.0f;
+                        o1 = fin ? o1 : 0.0f;
+                        o2 = fin ? o2 : 0.0f;
+                    }
+                    const float f1 = floorf(o1), f2 = floorf(o2);
+                    const float wl1 = o1 - f1, wm1 = o2 - f2;
+                    const int ca = c1 + (int)f1, ra = c2 + (int)f2;
+                    // Tile apron and |LL|, |MM| < h - 1 (FIPkernels.cu:389) as one
+                    // unsigned range per axis; on the left / top image edge an
+                    // integer position on the bound is outside.
+                    bool ok = fin && (unsigned)(ca - cxlo) <= (unsigned)cxn && (unsigned)(ra - rylo) <= (unsigned)ryn;
+                    if (cedge || redge)
+                        ok = ok && !(cedge && ca == cxlo && wl1 == 0.0f) && !(redge && ra == rylo && wm1 == 0.0f);
+                    if (ok) {
+                        const float vs = pv[k] * vscale;
+                        const float t0 = (1.0f - wl1) * vs, t1 = wl1 * vs, om = 1.0f - wm1;
+                        // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL); a
+                        // zero weight on an integer position adds an exact zero.
+                        int* p0 = acc + mad_u24((unsigned)(ra + 1), (unsigned)A, (unsigned)(ca + 1));
+                        acc_exact(p0, t0 * om);
+                        acc_exact(p0 + A, t0 * wm1);
+                        acc_exact(p0 + 1, t1 * om);
+                        acc_exact(p0 + A + 1, t1 * wm1);
                     }
                 }
             }
