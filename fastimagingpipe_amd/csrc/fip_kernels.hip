@@ -624,81 +624,81 @@ __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
 
 // ---------------------------------------------------------------- stage 2
 
+// Gather and C2R split of ROWS image rows in one pass: Z[k] = E[k] + i O[k],
+// E = K[k] + K~[k+M], O = (K[k] - K~[k+M]) e^{2 pi i k/G},
+// K~[k+M] = conj(K[M-k]) (k > 0), K[M] (k = 0).  A thread takes the pairs
+// (p, M-p) of every row, reads K[p] and K[M-p] straight from the K tiles
+// (zero beyond cmax) and writes Z[p], Z[M-p]; the pairs cover every index
+// 0..M-1.  wpost[M - p] = -conj(wpost[p]).  The loads of all the thread's
+// pairs are issued together (load), then paired into LDS (store).
 template <int M, int ROWS>
-__global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* __restrict__ kt,
-                                                                   SnapMeta* __restrict__ meta,
-                                                                   const float2* __restrict__ tw,
-                                                                   const float2* __restrict__ wpost,
-                                                                   const float* __restrict__ corr,
-                                                                   float* __restrict__ pre, int I)
-{
-    constexpr int NT = FftPlan<M>::nt;
-    constexpr int G = 2 * M;
-    constexpr int L = lds_len(M + 1);
-    __shared__ float2 zb[ROWS * L];
-    __shared__ unsigned shm[NT * ROWS / 64];
-    const int s = blockIdx.y;
-    const int y0 = xcd_remap(blockIdx.x, gridDim.x) * ROWS;
-    const int grp = threadIdx.x / NT, t = threadIdx.x % NT;
-    const int rmax = meta[s].rmax;
-    const int cmax = min(rmax, M);
-    float2* z = zb + grp * L;
-    const float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
+struct C2RGather {
+    static constexpr int NT = FftPlan<M>::nt * ROWS;          // threads of the workgroup
+    static constexpr int PB = (M / 2 + NT) / NT;               // pairs per thread: ceil((M/2+1)/NT)
+    float2 a[PB][ROWS], b[PB][ROWS], wv[PB];
 
-    // Gather and C2R split in one pass: Z[k] = E[k] + i O[k],
-    // E = K[k] + K~[k+M], O = (K[k] - K~[k+M]) e^{2 pi i k/G},
-    // K~[k+M] = conj(K[M-k]) (k > 0), K[M] (k = 0).  A thread takes the pairs
-    // (p, M-p) of every row of the workgroup, reads K[p] and K[M-p] straight
-    // from the K tiles (zero beyond cmax) and writes Z[p], Z[M-p]; the pairs
-    // cover every index 0..M-1.  wpost[M - p] = -conj(wpost[p]).
+    // kt0: K tile run of the first row of the group (row y0 of its snapshot).
+    __device__ __forceinline__ void load(const float2* __restrict__ kt0, const float2* __restrict__ wpost,
+                                         int cmax, int tid)
     {
-        constexpr int PB = 4;   // pairs per thread per pass, loads issued together
-        const float2* kt0 = ks + (size_t)(y0 / kKTile) * (M + 1) * kKTile + (y0 % kKTile);
-        for (int p0 = threadIdx.x; p0 <= M / 2; p0 += NT * ROWS * PB) {
-            float2 a[PB][ROWS], b[PB][ROWS], wv[PB];
 #pragma unroll
-            for (int u = 0; u < PB; ++u) {
-                const int p = p0 + u * NT * ROWS;
-                const bool ok = p <= M / 2;
-                wv[u] = ok ? wpost[p] : make_float2(0.0f, 0.0f);
+        for (int u = 0; u < PB; ++u) {
+            const int p = tid + u * NT;
+            const bool ok = p <= M / 2;
+            wv[u] = ok ? wpost[p] : make_float2(0.0f, 0.0f);
 #pragma unroll
-                for (int r = 0; r < ROWS; ++r) {
-                    a[u][r] = (ok && p <= cmax) ? kt0[(size_t)p * kKTile + r] : make_float2(0.0f, 0.0f);
-                    b[u][r] = (ok && M - p <= cmax) ? kt0[(size_t)(M - p) * kKTile + r] : make_float2(0.0f, 0.0f);
-                }
+            for (int r = 0; r < ROWS; ++r) {
+                a[u][r] = (ok && p <= cmax) ? kt0[(size_t)p * kKTile + r] : make_float2(0.0f, 0.0f);
+                b[u][r] = (ok && M - p <= cmax) ? kt0[(size_t)(M - p) * kKTile + r] : make_float2(0.0f, 0.0f);
             }
+        }
+    }
+
+    __device__ __forceinline__ void store(float2* zb, int tid) const
+    {
+        constexpr int L = lds_len(M + 1);
 #pragma unroll
-            for (int u = 0; u < PB; ++u) {
-                const int p = p0 + u * NT * ROWS;
-                if (p > M / 2) break;
+        for (int u = 0; u < PB; ++u) {
+            const int p = tid + u * NT;
+            if (p > M / 2) break;
 #pragma unroll
-                for (int r = 0; r < ROWS; ++r) {
-                    float2* zr = zb + r * L;
-                    const float2 av = a[u][r], bv = b[u][r];
-                    if (p == 0) {
-                        zr[lds_pad(0)] = cadd(cadd(av, bv), cmul_i(csub(av, bv)));
-                    } else {
-                        const float2 kp = cconj(bv);   // K~[p+M]
-                        zr[lds_pad(p)] = cadd(cadd(av, kp), cmul_i(cmul(csub(av, kp), wv[u])));
-                        if (p != M - p) {
-                            const float2 km = cconj(av);   // K~[M-p+M]
-                            const float2 wm = make_float2(-wv[u].x, wv[u].y);
-                            zr[lds_pad(M - p)] = cadd(cadd(bv, km), cmul_i(cmul(csub(bv, km), wm)));
-                        }
+            for (int r = 0; r < ROWS; ++r) {
+                float2* zr = zb + r * L;
+                const float2 av = a[u][r], bv = b[u][r];
+                if (p == 0) {
+                    zr[lds_pad(0)] = cadd(cadd(av, bv), cmul_i(csub(av, bv)));
+                } else {
+                    const float2 kp = cconj(bv);   // K~[p+M]
+                    zr[lds_pad(p)] = cadd(cadd(av, kp), cmul_i(cmul(csub(av, kp), wv[u])));
+                    if (p != M - p) {
+                        const float2 km = cconj(av);   // K~[M-p+M]
+                        const float2 wm = make_float2(-wv[u].x, wv[u].y);
+                        zr[lds_pad(M - p)] = cadd(cadd(bv, km), cmul_i(cmul(csub(bv, km), wm)));
                     }
                 }
             }
         }
     }
-    __syncthreads();
-    fft_inverse<M>(z, tw, t);
-    // d[2n] = Re z[n], d[2n+1] = Im z[n]; crop x in [-h, h), |.| and the
-    // column correction (FIPkernels.cu:256-287).
+};
+
+// First K tile run of image row y0 of snapshot s.
+template <int M>
+__device__ __forceinline__ const float2* c2r_rows(const float2* __restrict__ kt, int s, int y0, int I)
+{
+    const float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
+    return ks + (size_t)(y0 / kKTile) * (M + 1) * kKTile + (y0 % kKTile);
+}
+
+// d[2n] = Re z[n], d[2n+1] = Im z[n]; crop x in [-h, h), |.| and the column
+// correction (FIPkernels.cu:256-287) into one output row; returns the row
+// max of this thread.  Eight columns per thread per pass: the correction
+// loads of a pass are issued together.
+template <int M, int NT>
+__device__ __forceinline__ float c2r_output(const float2* z, const float* __restrict__ corr, float* out, int I, int t)
+{
+    constexpr int G = 2 * M;
     const int h = I / 2;
-    float* out = pre + (size_t)s * I * I + (size_t)(y0 + grp) * I;
     float lmax = 0.0f;
-    // Eight columns per thread per pass: the correction loads of a pass are
-    // issued together.
     for (int c0 = t; c0 < I; c0 += 8 * NT) {
         float cv[8];
 #pragma unroll
@@ -719,6 +719,34 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
             }
         }
     }
+    return lmax;
+}
+
+template <int M, int ROWS>
+__global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* __restrict__ kt,
+                                                                   SnapMeta* __restrict__ meta,
+                                                                   const float2* __restrict__ tw,
+                                                                   const float2* __restrict__ wpost,
+                                                                   const float* __restrict__ corr,
+                                                                   float* __restrict__ pre, int I)
+{
+    constexpr int NT = FftPlan<M>::nt;
+    constexpr int L = lds_len(M + 1);
+    __shared__ float2 zb[ROWS * L];
+    __shared__ unsigned shm[NT * ROWS / 64];
+    const int s = blockIdx.y;
+    const int y0 = xcd_remap(blockIdx.x, gridDim.x) * ROWS;
+    const int grp = threadIdx.x / NT, t = threadIdx.x % NT;
+    const int cmax = min(meta[s].rmax, M);
+    {
+        C2RGather<M, ROWS> gth;
+        gth.load(c2r_rows<M>(kt, s, y0, I), wpost, cmax, threadIdx.x);
+        gth.store(zb, threadIdx.x);
+    }
+    __syncthreads();
+    float2* z = zb + grp * L;
+    fft_inverse<M>(z, tw, t);
+    const float lmax = c2r_output<M, NT>(z, corr, pre + (size_t)s * I * I + (size_t)(y0 + grp) * I, I, t);
     const unsigned bm = block_max_u<NT * ROWS>(__float_as_uint(lmax), shm);
     if (threadIdx.x == 0) atomicMax(&meta[s].maxp_bits, bm);
 }
