@@ -14,7 +14,8 @@ import numpy as np
 
 __all__ = [
     "FipError", "lib", "library_path", "grid_size", "fipipe", "fipipe_lowp", "image",
-    "trigger", "pipe_device", "batch_device", "batch_host", "project", "collapse", "synth",
+    "trigger", "pipe_device", "batch_device", "batch_host", "project", "collapse", "ingest_table",
+    "synth",
     "set_profiling", "stage_times", "reset_stage_times", "autotune", "set_tuning",
     "debug_fft", "fits_read", "fits_write", "fits_write3", "Server", "STATUS",
 ]
@@ -72,6 +73,8 @@ _sig("fip_trigger", [_vp, _vp, _vp, _vp, _sz, _sz, _vp])
 _sig("fip_batch_device", [_vp, _vp, _i, _vp, _vp, _vp, _sz, _sz, _sz, _sz, _f, _sz, _vp, _vp])
 _sig("fip_project", [_vp, _sz, _vp, _sz, _vp, _vp])
 _sig("fip_collapse", [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp])
+_sig("fip_ingest_table", [_vp, _vp, _vp, _sz, _vp, _vp, _vp, _sz, _sz, _vp, _sz, _i, _i, ctypes.c_char_p,
+                          ctypes.c_char_p, _vp, _vp])
 _sig("fip_set_profiling", [_i])
 _sig("fip_stage_count", [])
 _sig("fip_stage_name", [_i], ctypes.c_char_p)
@@ -242,6 +245,37 @@ def collapse(xx, yy, flag0, flag3, w0, w3) -> np.ndarray:
     _check(lib.fip_collapse(*[_ptr(x) for x in a], _ptr(f0), _ptr(f3), _ptr(ww0), _ptr(ww3), n,
                             _ptr(ore), _ptr(oim)))
     return ore + 1j * oim
+
+
+def ingest_table(uvw, data, flag, freqs, time=None, file_index=None, weight=None,
+                 group_by_time: bool = False, workers: int = 0, prefix: str = "",
+                 combined: str | None = None):
+    """Measurement Set ingest from exported columns (fip_ingest_table).
+
+    uvw [nrows, 3] metres; data [nrows, nchan_data, npol] complex; flag of the
+    same shape (bool); freqs [nchan] Hz; time [nrows]; file_index [nrows]
+    (table of each row); weight [nrows, nchan_data, npol] or None.  Writes the
+    preprocessor's files; returns (snapshots, padded samples per snapshot).
+    """
+    u = _c(uvw, np.float64).reshape(-1, 3)
+    n = u.shape[0]
+    d = np.asarray(data)
+    if d.ndim != 3 or d.shape[0] != n:
+        raise ValueError("data must be [nrows, nchan, npol]")
+    nchan_data, npol = d.shape[1], d.shape[2]
+    dd = _c(np.stack([d.real, d.imag], axis=-1), np.float32)
+    fl = _c(np.asarray(flag).reshape(d.shape), np.uint8)
+    f = _c(np.atleast_1d(freqs), np.float64)
+    t = None if time is None else _c(np.asarray(time).reshape(n), np.float64)
+    fi = None if file_index is None else _c(np.asarray(file_index).reshape(n), np.int32)
+    w = None if weight is None else _c(np.asarray(weight).reshape(d.shape), np.float32)
+    ns, np_ = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _check(lib.fip_ingest_table(None if fi is None else _ptr(fi), None if t is None else _ptr(t), _ptr(u), n,
+                                _ptr(dd), _ptr(fl), None if w is None else _ptr(w), nchan_data, npol, _ptr(f),
+                                f.size, int(bool(group_by_time)), int(workers), prefix.encode(),
+                                None if combined is None else combined.encode(), ctypes.byref(ns),
+                                ctypes.byref(np_)))
+    return ns.value, np_.value
 
 
 def synth(nant: int, radius_m: float, nsnap: int, dt_s: float, freq_hz: float, dec_deg: float,

@@ -39,6 +39,7 @@ SOURCES = [
     "fip_synth.cpp",
     "fip_fits.cpp",
     "fip_serve.cpp",
+    "fip_ingest.cpp",
 ]
 
 

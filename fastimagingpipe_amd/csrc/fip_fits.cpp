@@ -5,10 +5,13 @@
 // BITPIX (8, 16, 32, 64, -32, -64), BSCALE/BZERO and NAXIS 0..3, converting to
 // float as fits_read_img(TFLOAT) does; other extensions are skipped by size.
 // Writes BITPIX -32 images (2-D or 3-D), replacing an existing file
-// (pipe_interface.cpp:102-123).
+// (pipe_interface.cpp:102-123), and files of named IMAGE extensions behind an
+// empty primary HDU (the preprocessor's combined input,
+// SVDPreprocessMain.cpp:181-240).
 #include "fip.h"
 
 #include "fip_context.hpp"
+#include "fip_fits.hpp"
 
 #include <algorithm>
 #include <cerrno>
@@ -271,6 +274,54 @@ void put_card(std::string& hdr, const std::string& key, const std::string& value
     hdr += c;
 }
 
+// A string value starting in column 11, as FITS fixed format requires.
+void put_str_card(std::string& hdr, const std::string& key, const std::string& value)
+{
+    std::string v = "'" + value;
+    while (v.size() < 9) v += ' ';   // strings are padded to at least 8 characters
+    v += "'";
+    char card[kCard + 1];
+    std::snprintf(card, sizeof(card), "%-8.8s= %s", key.c_str(), v.c_str());
+    std::string c(card);
+    c.resize(kCard, ' ');
+    hdr += c;
+}
+
+void end_header(std::string& hdr)
+{
+    std::string end("END");
+    end.resize(kCard, ' ');
+    hdr += end;
+    hdr.resize((hdr.size() + kBlock - 1) / kBlock * kBlock, ' ');
+}
+
+// Big-endian IEEE single data, padded to a whole block.
+std::vector<unsigned char> float_body(const float* data, size_t n)
+{
+    std::vector<unsigned char> body((n * 4 + kBlock - 1) / kBlock * kBlock, 0);
+    for (size_t i = 0; i < n; ++i) {
+        unsigned char b[4];
+        std::memcpy(b, &data[i], 4);
+        body[4 * i] = b[3];
+        body[4 * i + 1] = b[2];
+        body[4 * i + 2] = b[1];
+        body[4 * i + 3] = b[0];
+    }
+    return body;
+}
+
+int write_bytes(const char* path, const std::vector<std::vector<unsigned char>>& parts)
+{
+    std::remove(path);   // replace an existing file (pipe_interface.cpp:102-107)
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return fip::set_error(FIP_ERR_IO, std::string("cannot create ") + path);
+    bool ok = true;
+    for (const auto& p : parts) ok = ok && std::fwrite(p.data(), 1, p.size(), fp) == p.size();
+    const bool closed = std::fclose(fp) == 0;
+    if (!ok || !closed) return fip::set_error(FIP_ERR_IO, std::string("write failed: ") + path);
+    return FIP_OK;
+}
+
 int write_image(const char* path, const float* data, int naxis, const long* naxes)
 {
     if (!path || !data) return fip::set_error(FIP_ERR_ARG, "null argument");
@@ -284,30 +335,49 @@ int write_image(const char* path, const float* data, int naxis, const long* naxe
     put_card(hdr, "BITPIX", "-32", "IEEE single precision");
     put_card(hdr, "NAXIS", std::to_string(naxis));
     for (int k = 0; k < naxis; ++k) put_card(hdr, "NAXIS" + std::to_string(k + 1), std::to_string(naxes[k]));
-    std::string end("END");
-    end.resize(kCard, ' ');
-    hdr += end;
-    hdr.resize((hdr.size() + kBlock - 1) / kBlock * kBlock, ' ');
-    std::vector<unsigned char> body((n * 4 + kBlock - 1) / kBlock * kBlock, 0);
-    for (size_t i = 0; i < n; ++i) {
-        unsigned char b[4];
-        std::memcpy(b, &data[i], 4);
-        body[4 * i] = b[3];
-        body[4 * i + 1] = b[2];
-        body[4 * i + 2] = b[1];
-        body[4 * i + 3] = b[0];
-    }
-    std::remove(path);   // replace an existing file (pipe_interface.cpp:102-107)
-    FILE* fp = std::fopen(path, "wb");
-    if (!fp) return fip::set_error(FIP_ERR_IO, std::string("cannot create ") + path);
-    const bool ok = std::fwrite(hdr.data(), 1, hdr.size(), fp) == hdr.size() &&
-                    std::fwrite(body.data(), 1, body.size(), fp) == body.size();
-    const bool closed = std::fclose(fp) == 0;
-    if (!ok || !closed) return fip::set_error(FIP_ERR_IO, std::string("write failed: ") + path);
-    return FIP_OK;
+    end_header(hdr);
+    return write_bytes(path, {std::vector<unsigned char>(hdr.begin(), hdr.end()), float_body(data, n)});
 }
 
 }  // namespace
+
+namespace fip {
+
+int fits_write_extensions(const char* path, const FitsImage* ext, int n)
+{
+    if (!path || !ext || n <= 0) return set_error(FIP_ERR_ARG, "null argument");
+    std::vector<std::vector<unsigned char>> parts;
+    std::string hdr;
+    put_card(hdr, "SIMPLE", "T", "conforms to FITS standard");
+    put_card(hdr, "BITPIX", "8");
+    put_card(hdr, "NAXIS", "0");
+    put_card(hdr, "EXTEND", "T");
+    end_header(hdr);
+    parts.emplace_back(hdr.begin(), hdr.end());
+    for (int e = 0; e < n; ++e) {
+        const FitsImage& im = ext[e];
+        if (!im.data || im.naxis < 1 || im.naxis > 3) return set_error(FIP_ERR_ARG, "bad FITS extension");
+        size_t count = 1;
+        for (int k = 0; k < im.naxis; ++k) {
+            if (im.naxes[k] <= 0) return set_error(FIP_ERR_ARG, "FITS axes must be positive");
+            count *= (size_t)im.naxes[k];
+        }
+        hdr.clear();
+        put_str_card(hdr, "XTENSION", "IMAGE");
+        put_card(hdr, "BITPIX", "-32", "IEEE single precision");
+        put_card(hdr, "NAXIS", std::to_string(im.naxis));
+        for (int k = 0; k < im.naxis; ++k) put_card(hdr, "NAXIS" + std::to_string(k + 1), std::to_string(im.naxes[k]));
+        put_card(hdr, "PCOUNT", "0");
+        put_card(hdr, "GCOUNT", "1");
+        if (im.extname) put_str_card(hdr, "EXTNAME", im.extname);
+        end_header(hdr);
+        parts.emplace_back(hdr.begin(), hdr.end());
+        parts.push_back(float_body(im.data, count));
+    }
+    return write_bytes(path, parts);
+}
+
+}  // namespace fip
 
 extern "C" {
 

@@ -153,6 +153,36 @@ int fip_collapse(const float *xx_re, const float *xx_im, const float *yy_re,
                  const unsigned char *flag3, const float *w0, const float *w3,
                  size_t n, float *out_re, float *out_im);
 
+/*
+ * Measurement Set ingest from exported table columns (SVDPreprocessMain.cpp:
+ * 337-546, 548-640, without the casacore reader: the columns come from an
+ * external converter such as python-casacore's getcol).
+ *   file_index [nrows]: table (MS file) of each row, or NULL (one table);
+ *   time [nrows]: TIME (needed with group_by_time);
+ *   uvw [nrows][3] metres; data [nrows][nchan_data][npol][2] float (re, im);
+ *   flag [nrows][nchan_data][npol] (0/1); weight [nrows][nchan_data][npol]
+ *   (WEIGHT_SPECTRUM) or NULL (all 1); freqs [nchan] Hz (CHAN_FREQ of every
+ *   SPECTRAL_WINDOW row, concatenated).
+ * Snapshots: tables in ascending file_index, each one snapshot or, with
+ * group_by_time, one per distinct TIME in ascending order (rows kept in row
+ * order).  Per snapshot, XX = pol 0 and YY = pol 3 of channel c read DATA
+ * channel c % nchan_data; samples are channel-major; UVW and frequencies are
+ * rounded to float as the reference's reader stores them; then fip_collapse
+ * and fip_project.  `workers` snapshots are preprocessed concurrently (0:
+ * min(4, hardware threads, snapshots)).  Snapshots are zero-padded to the
+ * largest sample count.  Writes <prefix>Bin.fits, Vin.fits, Visreal.fits,
+ * Visimag.fits and the 3-extension combined file `combined` (NULL or "":
+ * <prefix>input.fits).  *nsnap_out / *nsamp_out (optional) get the snapshot
+ * count and padded samples per snapshot.  FIP_ERR_ARG for npol < 4.
+ */
+int fip_ingest_table(const int *file_index, const double *time,
+                     const double *uvw, size_t nrows, const float *data,
+                     const unsigned char *flag, const float *weight,
+                     size_t nchan_data, size_t npol, const double *freqs,
+                     size_t nchan, int group_by_time, int workers,
+                     const char *prefix, const char *combined,
+                     size_t *nsnap_out, size_t *nsamp_out);
+
 /* ---- per-stage timing (hipEvents on the launching stream) -------------- */
 /* Enable or disable per-stage event timing for subsequent calls. */
 int fip_set_profiling(int on);

@@ -1,0 +1,152 @@
+"""Measurement Set ingest from exported columns (SVDPreprocessMain.cpp:337-640).
+
+The layout test pins the reference's conventions with marker values: the YY
+samples are flagged and the weights are 1, so the weighted collapse returns XX
+exactly, and XX of row r, DATA channel d is 100 r + d.  Each output sample then
+names the row and channel it came from: TIME groups in ascending order within
+each table, rows in row order, channel-major samples, SPECTRAL_WINDOW channel c
+reading DATA channel c % nchan_data, zero padding to the largest snapshot.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _table(seed=3, nchan_data=2):
+    rng = np.random.default_rng(seed)
+    # table 0: TIMEs 5.0 and 3.0 interleaved (rows 0..6); table 1: TIME 1.0 (rows 7..9)
+    time = np.array([5.0, 3.0, 5.0, 3.0, 3.0, 5.0, 5.0, 1.0, 1.0, 1.0])
+    fidx = np.array([0] * 7 + [1] * 3, np.int32)
+    n = time.size
+    uvw = rng.uniform(-3000.0, 3000.0, (n, 3))
+    data = np.zeros((n, nchan_data, 4), np.complex64)
+    for r in range(n):
+        for d in range(nchan_data):
+            data[r, d, 0] = 100.0 * r + d + 0.25j
+            data[r, d, 3] = -7.0 + 1j
+            data[r, d, 1] = data[r, d, 2] = 99.0
+    flag = np.zeros((n, nchan_data, 4), bool)
+    flag[:, :, 3] = True
+    freqs = np.array([140e6, 140.2e6, 140.4e6])
+    return time, fidx, uvw, data, flag, freqs
+
+
+def _read(fip, path, ext=None):
+    return fip.fits_read(path, ext) if ext else fip.fits_read(path)
+
+
+def test_ingest_rejects_fewer_than_four_polarisations():
+    import fastimagingpipe_amd as F
+    time, fidx, uvw, data, flag, freqs = _table()
+    with pytest.raises(F.FipError) as e:
+        F.ingest_table(uvw, data[:, :, :2], flag[:, :, :2], freqs, time=time, group_by_time=True)
+    assert e.value.status == -1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("by_time", [True, False])
+def test_ingest_layout_grouping_and_padding(fip, tmp_path, by_time):
+    time, fidx, uvw, data, flag, freqs = _table()
+    prefix = str(tmp_path / "p_")
+    ns, np_ = fip.ingest_table(uvw, data, flag, freqs, time=time, file_index=fidx, group_by_time=by_time,
+                               workers=2, prefix=prefix)
+    groups = [[1, 3, 4], [0, 2, 5, 6], [7, 8, 9]] if by_time else [[0, 1, 2, 3, 4, 5, 6], [7, 8, 9]]
+    nchan = freqs.size
+    assert ns == len(groups) and np_ == max(len(g) for g in groups) * nchan
+    vis = _read(fip, prefix + "input.fits", "VISIBILITIES")
+    rco = _read(fip, prefix + "input.fits", "R-COORDINATES")
+    vin = _read(fip, prefix + "input.fits", "TRANSFORMATION MATRICES")
+    assert vis.shape == (ns, np_, 2) and rco.shape == (ns, np_, 2) and vin.shape == (ns, 3, 3)
+    for k, rows in enumerate(groups):
+        nr = len(rows)
+        want = np.zeros(np_, np.complex64)
+        for c in range(nchan):
+            for i, r in enumerate(rows):
+                want[i + c * nr] = 100.0 * r + c % 2 + 0.25j
+        got = vis[k, :, 0] + 1j * vis[k, :, 1]
+        assert np.array_equal(got, want), k
+        # projection of the group's rows, UVW and frequencies as float
+        b, v = fip.project(uvw[rows].astype(np.float32).astype(np.float64),
+                           freqs.astype(np.float32).astype(np.float64))
+        assert np.array_equal(rco[k, :nr * nchan], b) and not rco[k, nr * nchan:].any()
+        assert np.array_equal(vin[k], v)
+    # the 2-D files carry the same numbers (SVDPreprocessMain.cpp:617-620)
+    assert np.array_equal(_read(fip, prefix + "Bin.fits").reshape(-1, 2), rco.reshape(-1, 2))
+    assert np.array_equal(_read(fip, prefix + "Vin.fits").reshape(-1, 3), vin.reshape(-1, 3))
+    assert np.array_equal(_read(fip, prefix + "Visreal.fits").reshape(-1), vis[:, :, 0].reshape(-1))
+    assert np.array_equal(_read(fip, prefix + "Visimag.fits").reshape(-1), vis[:, :, 1].reshape(-1))
+
+
+@pytest.mark.gpu
+def test_ingest_weights_and_flags_match_collapse(fip, tmp_path):
+    rng = np.random.default_rng(11)
+    n, nd = 40, 3
+    time = np.repeat([10.0, 20.0], n // 2)
+    uvw = rng.uniform(-5000.0, 5000.0, (n, 3))
+    data = (rng.standard_normal((n, nd, 4)) + 1j * rng.standard_normal((n, nd, 4))).astype(np.complex64)
+    flag = rng.random((n, nd, 4)) < 0.2
+    weight = rng.uniform(0.1, 2.0, (n, nd, 4)).astype(np.float32)
+    freqs = np.array([150e6, 151e6, 152e6])
+    comb = str(tmp_path / "c.fits")
+    ns, np_ = fip.ingest_table(uvw, data, flag, freqs, time=time, weight=weight, group_by_time=True,
+                               prefix=str(tmp_path / "q_"), combined=comb)
+    assert (ns, np_) == (2, (n // 2) * 3)
+    vis = _read(fip, comb, "VISIBILITIES")
+    for k in range(2):
+        rows = np.arange(k * n // 2, (k + 1) * n // 2)
+        idx = [(r, c) for c in range(3) for r in rows]   # channel-major
+        xx = np.array([data[r, c, 0] for r, c in idx])
+        yy = np.array([data[r, c, 3] for r, c in idx])
+        f0 = np.array([flag[r, c, 0] for r, c in idx])
+        f3 = np.array([flag[r, c, 3] for r, c in idx])
+        w0 = np.array([weight[r, c, 0] for r, c in idx])
+        w3 = np.array([weight[r, c, 3] for r, c in idx])
+        want = fip.collapse(xx, yy, f0, f3, w0, w3)
+        assert np.array_equal(vis[k, :, 0] + 1j * vis[k, :, 1], want.astype(np.complex64))
+
+
+@pytest.mark.gpu
+def test_ingest_cli_feeds_the_combined_pipeline(fip, tmp_path):
+    """Two exported tables -> python -m fastimagingpipe_amd.ingest -> sharedlibrary_gpu --combined."""
+    rng = np.random.default_rng(5)
+    nant = 24
+    ant = rng.uniform(-800.0, 800.0, (nant, 3)) * [1, 1, 0.05]
+    i, j = np.triu_indices(nant, 1)
+    paths = []
+    for k in range(2):
+        times = [100.0 + 2 * k, 101.0 + 2 * k]
+        rows = []
+        for t in times:
+            h = 2e-3 * (t - 100.0)
+            rot = np.array([[np.cos(h), -np.sin(h), 0], [np.sin(h), np.cos(h), 0], [0, 0, 1]])
+            rows.append(((ant[i] - ant[j]) @ rot.T, np.full(i.size, t)))
+        uvw = np.concatenate([r[0] for r in rows])
+        tt = np.concatenate([r[1] for r in rows])
+        n = uvw.shape[0]
+        data = (rng.standard_normal((n, 1, 4)) + 1j * rng.standard_normal((n, 1, 4))).astype(np.complex64)
+        flag = np.zeros((n, 1, 4), bool)
+        p = str(tmp_path / f"t{k}.npz")
+        np.savez(p, TIME=tt, UVW=uvw, DATA=data, FLAG=flag, CHAN_FREQ=np.array([[140e6]]))
+        paths.append(p)
+    comb = str(tmp_path / "input.fits")
+    r = subprocess.run([sys.executable, "-m", "fastimagingpipe_amd.ingest", "--group-by-time", "--snapshot-workers",
+                        "2", "--combined-output", comb, "--output-prefix", str(tmp_path / "x_"), *paths],
+                       cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    vis = _read(fip, comb, "VISIBILITIES")
+    assert vis.shape[0] == 4
+    out = str(tmp_path / "o.fits")
+    r = subprocess.run([os.path.join(ROOT, "sharedlibrary_gpu"), "--combined", comb, "256", "2e-4", "16", out],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = fip.fits_read(out).reshape(16, 16)
+    rco = _read(fip, comb, "R-COORDINATES")
+    vin = _read(fip, comb, "TRANSFORMATION MATRICES")
+    nb = vis.shape[1]
+    ref = fip.fipipe(vis[:3, :, 0], vis[:3, :, 1], rco[:3], vin[:3], nb, 256, 3, 2e-4, 16)
+    assert np.array_equal(got, ref)
