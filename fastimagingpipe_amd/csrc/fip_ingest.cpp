@@ -9,6 +9,7 @@
 #include "fip_fits.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <deque>
 #include <future>
 #include <map>
@@ -119,6 +120,9 @@ extern "C" int fip_ingest_table(const int* file_index, const double* time, const
         if (nchan_data == 0 || nchan == 0) return set_error(FIP_ERR_ARG, "DATA and CHAN_FREQ need channels");
         if (npol < 4) return set_error(FIP_ERR_ARG, "DATA has fewer than 4 polarisations");
         if (workers < 0) return set_error(FIP_ERR_ARG, "workers must be >= 0");
+        if (group_by_time)
+            for (size_t r = 0; r < nrows; ++r)
+                if (!std::isfinite(time[r])) return set_error(FIP_ERR_ARG, "non-finite TIME in row " + std::to_string(r));
         const Table T{uvw, data, flag, weight, nchan_data, npol, freqs, nchan};
         const std::vector<Group> groups = group_rows(file_index, time, nrows, group_by_time != 0);
         const size_t S = groups.size();

@@ -48,6 +48,15 @@ def test_ingest_rejects_fewer_than_four_polarisations():
     assert e.value.status == -1
 
 
+def test_ingest_rejects_non_finite_time():
+    import fastimagingpipe_amd as F
+    time, fidx, uvw, data, flag, freqs = _table()
+    time[4] = np.nan
+    with pytest.raises(F.FipError) as e:
+        F.ingest_table(uvw, data, flag, freqs, time=time, group_by_time=True)
+    assert e.value.status == -1
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("by_time", [True, False])
 def test_ingest_layout_grouping_and_padding(fip, tmp_path, by_time):
