@@ -256,6 +256,7 @@ int fip_image(const float* vis_re, const float* vis_im, const float* bin, const 
         if (int rc = run_imaging(*C, *P, C->work, in, S, (int)nbl, cell_size, nullptr, (int)nbl,
                                  C->tuning(image_size), st))
             return rc;
+        FIP_HIP(launch_max_finalize(C->work.view(nbl), S, st));
         FIP_HIP(hipStreamSynchronize(st));
         if (int rc = C->collect_stages()) return rc;
         const size_t ib = (size_t)S * image_size * image_size * sizeof(float);
@@ -296,6 +297,11 @@ int fip_trigger(const float* d1, const float* d2, const float* d3, const float m
         std::memset(meta, 0, sizeof(meta));
         for (int s = 0; s < 3; ++s) std::memcpy(&meta[s].maxd_bits, &maxall[s], 4);
         FIP_HIP(hipMemcpy(C->work.meta.p, meta, sizeof(meta), hipMemcpyHostToDevice));
+        // The trigger reads the image maxima from the spread slots: slot 0
+        // holds maxall[s], the others zero.
+        std::vector<unsigned> slots(3 * kPmaxPer, 0u);
+        for (int s = 0; s < 3; ++s) std::memcpy(&slots[(size_t)s * kPmaxPer + kMaxSlots * kSlotStride], &maxall[s], 4);
+        FIP_HIP(hipMemcpy(C->work.pmax.p, slots.data(), slots.size() * sizeof(unsigned), hipMemcpyHostToDevice));
         const size_t un = image_size / unit_size;
         FIP_HIP(C->out.ensure(un * un * sizeof(float)));
         if (int rc = run_trigger(*C, *P, C->work, 1, C->win.as<int>(), 1, (int)unit_size, C->out.as<float>(),

@@ -83,7 +83,7 @@ size_t Work::bytes(int S, size_t stride, const PlanTables& p) const
     const size_t nvis = s * stride;
     return nvis * sizeof(VisRec) * 2 + nvis * sizeof(int2) + s * p.G * 4 + s * (p.G + 1) * 4 +
            s * sizeof(SnapMeta) + s * sizeof(SnapGeom) + s * (size_t)p.I * (p.M + 1) * 8 +
-           2 * s * (size_t)p.I * p.I * 4 + 256;
+           2 * s * (size_t)p.I * p.I * 4 + 256 + s * kPmaxPer * sizeof(unsigned);
 }
 
 int Work::ensure(int S, size_t stride, const PlanTables& p)
@@ -101,6 +101,7 @@ int Work::ensure(int S, size_t stride, const PlanTables& p)
     FIP_HIP(pre.ensure(s * (size_t)p.I * p.I * sizeof(float)));
     FIP_HIP(img.ensure(s * (size_t)p.I * p.I * sizeof(float)));
     FIP_HIP(work.ensure(256));
+    FIP_HIP(pmax.ensure(s * kPmaxPer * sizeof(unsigned)));
     return FIP_OK;
 }
 
@@ -118,13 +119,15 @@ WorkView Work::view(size_t stride) const
     v.pre = pre.as<float>();
     v.img = img.as<float>();
     v.work = work.as<unsigned>();
+    v.pmax = pmax.as<unsigned>();
     v.stride = stride;
     return v;
 }
 
 void Work::release()
 {
-    for (DevBuf* b : {&rec, &bkslot, &sorted, &hist, &bstart, &meta, &geom, &kt, &pre, &img, &work}) b->release();
+    for (DevBuf* b : {&rec, &bkslot, &sorted, &hist, &bstart, &meta, &geom, &kt, &pre, &img, &work, &pmax})
+        b->release();
 }
 
 // ------------------------------------------------------------- context

@@ -47,7 +47,7 @@ struct Plan {
 
 // Work buffers sized for S snapshots of one image size and stride.
 struct Work {
-    DevBuf rec, bkslot, sorted, hist, bstart, meta, geom, kt, pre, img, work;
+    DevBuf rec, bkslot, sorted, hist, bstart, meta, geom, kt, pre, img, work, pmax;
     WorkView view(size_t stride) const;
     int ensure(int S, size_t stride, const PlanTables& p);
     size_t bytes(int S, size_t stride, const PlanTables& p) const;

@@ -30,14 +30,30 @@ struct VisRec {
     float vi;
 };
 
+// Image maxima are gathered in spread slots: per snapshot, kMaxSlots slots
+// for the corrected image before re-projection (slot set 0) and kMaxSlots for
+// the re-projected image (set 1).  Workgroup g updates slot g % kMaxSlots, so
+// about 1/kMaxSlots of a launch's workgroups contend for one address (one
+// address for every workgroup serialised the splat's tail: 4096 atomics per
+// image cost ~40 us); consumers take the max of the slots.  Contention is per
+// address, not per cache line (slots 128 bytes apart measured the same for
+// the writers), so the slots are packed and a wave reads them in two lines.
+constexpr int kMaxSlots = 64;
+constexpr int kSlotStride = 1;                                   // unsigned per slot
+constexpr size_t kPmaxPer = (size_t)2 * kMaxSlots * kSlotStride;   // unsigned per snapshot
+__host__ __device__ inline unsigned* max_slot(unsigned* pm, int s, int set, int slot)
+{
+    return pm + (size_t)s * kPmaxPer + ((size_t)set * kMaxSlots + slot) * kSlotStride;
+}
+
 // Per-snapshot scalars produced and consumed on the device.
 struct SnapMeta {
     int ngrid;            // visibilities with at least one grid tap
     int rmax;             // largest |column tap|: stage 1 covers c = 0..rmax
     int r1max;            // largest |row tap|: the int64 gridding band
     unsigned maxv_bits;   // max(|re|,|im|) of weighted visibilities (float bits)
-    unsigned maxp_bits;   // max of the corrected image before re-projection
-    unsigned maxd_bits;   // max of the re-projected image (maxall entry)
+    unsigned maxp_bits;   // max of the corrected image before re-projection (k_max_finalize)
+    unsigned maxd_bits;   // max of the re-projected image, maxall entry (k_trigger, k_max_finalize)
     int norm_n;           // N used by the splat's 1/N (num_baselines)
     int nvis;             // visibilities present in this snapshot
 };
@@ -92,6 +108,7 @@ struct WorkView {
     float* pre;        // [S][I][I] corrected image before re-projection
     float* img;        // [S][I][I] re-projected images
     unsigned* work;    // [1] persistent-kernel work counter
+    unsigned* pmax;    // [S][kPmaxPer] spread slots of the image maxima
     size_t stride;
 };
 
@@ -133,6 +150,10 @@ hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tun
 // Trigger over nwin windows; win: device int[3*nwin] of snapshot slots in img.
 hipError_t launch_trigger(const WorkView& w, const PlanTables& p, const int* win, int nwin,
                           int tile, float* out, hipStream_t st);
+
+// meta[s].maxp_bits / maxd_bits from the spread slots, for host readers of
+// imaging results without a trigger launch.
+hipError_t launch_max_finalize(const WorkView& w, int S, hipStream_t st);
 
 // Inverse FFT of `count` independent length-n arrays in place (self-check).
 hipError_t launch_fft_check(int n, float2* data, const float2* tw, int count, hipStream_t st);

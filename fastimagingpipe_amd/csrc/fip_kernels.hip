@@ -132,10 +132,11 @@ __device__ __forceinline__ long long to_fixed(float x)
 
 __global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ nvis, int n_const,
                         const int* __restrict__ norm_n, int norm_const, SnapMeta* meta, SnapGeom* geom,
-                        int* hist, unsigned* work, int S, int G, int I, float cell)
+                        int* hist, unsigned* work, unsigned* pmax, int S, int G, int I, float cell)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < S * G) hist[i] = 0;
+    if (i < S * 2 * kMaxSlots) *max_slot(pmax, i / (2 * kMaxSlots), 0, i % (2 * kMaxSlots)) = 0u;
     if (i == 0) work[0] = 0u;
     if (i < S) {
         SnapMeta m;
@@ -724,7 +725,8 @@ __device__ __forceinline__ float c2r_output(const float2* z, const float* __rest
 
 template <int M, int ROWS>
 __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* __restrict__ kt,
-                                                                   SnapMeta* __restrict__ meta,
+                                                                   const SnapMeta* __restrict__ meta,
+                                                                   unsigned* __restrict__ pmax,
                                                                    const float2* __restrict__ tw,
                                                                    const float2* __restrict__ wpost,
                                                                    const float* __restrict__ corr,
@@ -748,7 +750,7 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
     fft_inverse<M>(z, tw, t);
     const float lmax = c2r_output<M, NT>(z, corr, pre + (size_t)s * I * I + (size_t)(y0 + grp) * I, I, t);
     const unsigned bm = block_max_u<NT * ROWS>(__float_as_uint(lmax), shm);
-    if (threadIdx.x == 0) atomicMax(&meta[s].maxp_bits, bm);
+    if (threadIdx.x == 0) atomicMax(max_slot(pmax, s, 0, (int)(blockIdx.x % kMaxSlots)), bm);
 }
 
 // ----------------------------------------------------------------- splat
@@ -823,9 +825,9 @@ __device__ __forceinline__ void acc_exact(int* a, float x)
 // loads and position chains in flight; at most 64 VGPRs keep eight waves
 // per SIMD resident.
 template <int TS>
-__global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre, SnapMeta* __restrict__ meta,
-                                               const SnapGeom* __restrict__ geom, float* __restrict__ img,
-                                               int I)
+__global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre, const SnapMeta* __restrict__ meta,
+                                               const SnapGeom* __restrict__ geom, unsigned* __restrict__ pmax,
+                                               float* __restrict__ img, int I)
 {
     constexpr int NT = 256;
     constexpr int NW = NT / 64;           // waves
@@ -841,6 +843,7 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
     __shared__ double qb[6];
     __shared__ int bb[4];
     __shared__ unsigned shm[NT / 64];
+    __shared__ unsigned smaxp;
     const int s = blockIdx.y;
     const int ntx = I / TS;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -910,6 +913,12 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
         int4* a4 = reinterpret_cast<int4*>(acc);
         for (int q = tid; q < A * A / 4; q += NT) a4[q] = make_int4(0, 0, 0, 0);
     }
+    if (wid == 1) {
+        // Max of the corrected image from its slots (one per lane), while
+        // thread 0 computes the box.
+        const unsigned v = wave_max_u(*max_slot(pmax, s, 0, lane));
+        if (lane == 0) smaxp = v;
+    }
     __syncthreads();
     const int i0 = bb[0], i1 = bb[1], j0 = bb[2], j1 = bb[3];
     // Accepted target offsets of a footprint's top-left corner: the apron
@@ -924,7 +933,8 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
     // A pixel sums at most ccap contributions of at most max(pre)/N each
     // (1/N of FIPkernels.cu:391); 2^k * ccap * max(pre)/N < 2^29.
     int e = 0;
-    frexpf(fmaxf((float)g.ccap * (__uint_as_float(m.maxp_bits) / (float)m.norm_n), 1e-30f), &e);
+    const unsigned maxp_bits = smaxp;
+    frexpf(fmaxf((float)g.ccap * (__uint_as_float(maxp_bits) / (float)m.norm_n), 1e-30f), &e);
     const int ks = min(29 - e, 120);
     const float vscale = ldexpf(1.0f, ks) / (float)m.norm_n;
     const float iscale = ldexpf(1.0f, -ks);
@@ -1094,16 +1104,16 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
         lmax = fmaxf(lmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
     }
     const unsigned bmx = block_max_u<NT>(__float_as_uint(lmax), shm);
-    if (tid == 0) atomicMax(&meta[s].maxd_bits, bmx);
+    if (tid == 0) atomicMax(max_slot(pmax, s, 1, (int)(blockIdx.x % kMaxSlots)), bmx);
 }
 
 // --------------------------------------------------------------- trigger
 
 // One wave per tile (four tiles per workgroup): per-lane partial sums over
 // 16-byte loads, then wave reductions only (FIPkernels.cu:499-552).
-__global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, const SnapMeta* __restrict__ meta,
-                                                 const int* __restrict__ win, int I, int T,
-                                                 float* __restrict__ out)
+__global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, SnapMeta* __restrict__ meta,
+                                                 unsigned* __restrict__ pmax, const int* __restrict__ win, int I,
+                                                 int T, float* __restrict__ out)
 {
     const int w = blockIdx.y;
     const int un = I / T;
@@ -1112,6 +1122,9 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
     if (tile >= un * un) return;   // whole wave
     const int ti = tile / un, tj = tile % un;
     const int s1 = win[3 * w], s2 = win[3 * w + 1], s3 = win[3 * w + 2];
+    // Image maxima from the spread slots (one slot per lane).
+    const unsigned q1 = *max_slot(pmax, s1, 1, lane), q2 = *max_slot(pmax, s2, 1, lane),
+                   q3 = *max_slot(pmax, s3, 1, lane);
     const float* d1 = img + (size_t)s1 * I * I;
     const float* d2 = img + (size_t)s2 * I * I;
     const float* d3 = img + (size_t)s3 * I * I;
@@ -1160,9 +1173,15 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
     sum = wave_sum_f(sum);
     mx = wave_max_f(mx);
     sr = wave_sum_f(sr);
+    const unsigned m1b = wave_max_u(q1), m2b = wave_max_u(q2), m3b = wave_max_u(q3);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the reduced maxima for host readers (every window writes the same values)
+        meta[s1].maxd_bits = m1b;
+        meta[s2].maxd_bits = m2b;
+        meta[s3].maxd_bits = m3b;
+    }
     if (lane == 0) {
-        const float m3 = fmaxf(__uint_as_float(meta[s1].maxd_bits),
-                               fmaxf(__uint_as_float(meta[s2].maxd_bits), __uint_as_float(meta[s3].maxd_bits)));
+        const float m3 = fmaxf(__uint_as_float(m1b), fmaxf(__uint_as_float(m2b), __uint_as_float(m3b)));
         const float Tf = (float)T;
         // FIPkernels.cu:550
         out[(size_t)w * un * un + tile] = 1.0f - (sum / Tf / Tf) * mx * (sr / Tf / Tf) / m3 / m3;
@@ -1181,9 +1200,9 @@ const char* stage_name(int st)
 hipError_t launch_reset(const InputView& in, const WorkView& w, const PlanTables& p, int S, float cell,
                         const int* norm_n_dev, int norm_n_const, hipStream_t st)
 {
-    const int total = max(S * p.G, S);
+    const int total = max(S * p.G, S * 2 * kMaxSlots);
     hipLaunchKernelGGL(k_reset, dim3((total + 255) / 256), dim3(256), 0, st, in.vin, in.nvis, in.n_const,
-                       norm_n_dev, norm_n_const, w.meta, w.geom, w.hist, w.work, S, p.G, p.I, cell);
+                       norm_n_dev, norm_n_const, w.meta, w.geom, w.hist, w.work, w.pmax, S, p.G, p.I, cell);
     return hipGetLastError();
 }
 
@@ -1275,8 +1294,8 @@ template <int M, int ROWS>
 static void stage2_dispatch(const WorkView& w, const PlanTables& p, int S, hipStream_t st)
 {
     constexpr int NT = FftPlan<M>::nt * ROWS;
-    hipLaunchKernelGGL((k_stage2<M, ROWS>), dim3(p.I / ROWS, S), dim3(NT), 0, st, w.kt, w.meta, p.tw_m, p.wpost,
-                       p.corr, w.pre, p.I);
+    hipLaunchKernelGGL((k_stage2<M, ROWS>), dim3(p.I / ROWS, S), dim3(NT), 0, st, w.kt, w.meta, w.pmax, p.tw_m,
+                       p.wpost, p.corr, w.pre, p.I);
 }
 
 template <int M>
@@ -1319,10 +1338,10 @@ hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tun
 {
     if (tu.splat_tile >= 64 && p.I % 64 == 0) {
         const int nt = (p.I / 64) * (p.I / 64);
-        hipLaunchKernelGGL(k_splat<64>, dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.img, p.I);
+        hipLaunchKernelGGL(k_splat<64>, dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img, p.I);
     } else {
         const int nt = (p.I / 32) * (p.I / 32);
-        hipLaunchKernelGGL(k_splat<32>, dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.img, p.I);
+        hipLaunchKernelGGL(k_splat<32>, dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img, p.I);
     }
     return hipGetLastError();
 }
@@ -1332,8 +1351,27 @@ hipError_t launch_trigger(const WorkView& w, const PlanTables& p, const int* win
 {
     const int un = p.I / tile;
     if (un <= 0 || nwin <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_trigger, dim3((un * un + 3) / 4, nwin), dim3(256), 0, st, w.img, w.meta, win, p.I, tile,
-                       out);
+    hipLaunchKernelGGL(k_trigger, dim3((un * un + 3) / 4, nwin), dim3(256), 0, st, w.img, w.meta, w.pmax, win, p.I,
+                       tile, out);
+    return hipGetLastError();
+}
+
+// One wave per snapshot: the slot maxima into meta.
+__global__ __launch_bounds__(64) void k_max_finalize(SnapMeta* __restrict__ meta, unsigned* __restrict__ pmax)
+{
+    const int s = blockIdx.x, lane = threadIdx.x;
+    const unsigned mp = wave_max_u(*max_slot(pmax, s, 0, lane));
+    const unsigned md = wave_max_u(*max_slot(pmax, s, 1, lane));
+    if (lane == 0) {
+        meta[s].maxp_bits = mp;
+        meta[s].maxd_bits = md;
+    }
+}
+
+hipError_t launch_max_finalize(const WorkView& w, int S, hipStream_t st)
+{
+    if (S <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_max_finalize, dim3(S), dim3(64), 0, st, w.meta, w.pmax);
     return hipGetLastError();
 }
 
