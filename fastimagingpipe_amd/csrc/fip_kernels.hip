@@ -7,16 +7,17 @@
 //            histogram with per-visibility slots
 //   scan     exclusive scan of the bucket histogram
 //   scatter  visibilities into bucket order (counting sort by grid column)
-//   stage1   per uv column c >= 0: deterministic int64 LDS gridding of the
+//   stage1   per uv column c >= 0: deterministic integer LDS gridding of the
 //            Hermitian-folded grid column (FIPkernels.cu:197-245) and its
-//            length-g inverse FFT along the row axis, cropped and multiplied
-//            by the row correction (FIPkernels.cu:256-287, first axis)
+//            length-g inverse FFT along the row axis, cropped to I rows
 //   stage2   per image row: complex-to-real length-g inverse FFT along the
-//            column axis via a length-g/2 complex FFT, cropped, |.| and
-//            column correction -> corrected image (FIPkernels.cu:256-287)
-//   splat    per target tile: SIN/affine re-projection evaluated in FP64
-//            positions and the bilinear forward splat accumulated in int64
-//            LDS, plus the image maximum (FIPkernels.cu:289-497)
+//            column axis via a length-g/2 complex FFT, cropped, |.| and the
+//            separable grid correction -> corrected image
+//            (FIPkernels.cu:256-287), plus its maximum (spread slots)
+//   splat    per target tile: affine + SIN re-projection (FP64 per group of
+//            sources, FP32 within it) and the bilinear forward splat
+//            accumulated exactly in int32 LDS, plus the image maximum
+//            (FIPkernels.cu:289-497)
 //   trigger  per tile and window of three images: tLISI (FIPkernels.cu:415-552)
 //
 // The image identity the stages rely on: with the reference's modulation
@@ -458,7 +459,7 @@ __device__ __forceinline__ int4 column_ranges(const int* __restrict__ bst, int c
 template <int G, int KR>
 __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRec* __restrict__ sorted,
                                               const SnapMeta& m,
-                                              const float2* __restrict__ tw, const float* __restrict__ corr,
+                                              const float2* __restrict__ tw,
                                               float2* __restrict__ kt, size_t stride, int I, float2* fb,
                                               unsigned* mass, float* sc, float* isc)
 {
@@ -547,26 +548,15 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
     }
     __syncthreads();
     fft_inverse<G>(fb, tw, t);
-    // Crop y in [-h, h) and apply the row correction 1/(c(|y|) norm).
+    // Crop y in [-h, h).  The row correction 1/(c(|y|) norm) is a factor per
+    // image row, applied by the row pass (one load per row there instead of
+    // one per row and column here).
     const int h = I / 2;
     float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
-    for (int y0 = t; y0 < I; y0 += 8 * NT) {
-        float cv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int y = y0 + u * NT - h;
-            cv[u] = (y0 + u * NT < I) ? corr[y < 0 ? -y : y] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int yp = y0 + u * NT;
-            if (yp < I) {
-                const int y = yp - h;
-                const float2 v = fb[lds_pad(y < 0 ? y + G : y)];
-                ks[((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)] =
-                    make_float2(v.x * cv[u], v.y * cv[u]);
-            }
-        }
+#pragma unroll 4
+    for (int yp = t; yp < I; yp += NT) {
+        const int y = yp - h;
+        ks[((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)] = fb[lds_pad(y < 0 ? y + G : y)];
     }
 }
 
@@ -575,7 +565,6 @@ __global__ __launch_bounds__(FftPlan<G>::nt, (FftPlan<G>::nt >= 1024 ? 1 : 2)) v
                                                             const int* __restrict__ bstart,
                                                             const SnapMeta* __restrict__ meta,
                                                             const float2* __restrict__ tw,
-                                                            const float* __restrict__ corr,
                                                             float2* __restrict__ kt, size_t stride, int I)
 {
     __shared__ float2 fb[lds_len(G)];
@@ -591,7 +580,7 @@ __global__ __launch_bounds__(FftPlan<G>::nt, (FftPlan<G>::nt >= 1024 ? 1 : 2)) v
     const int4 rg = column_ranges<G>(bstart + (size_t)s * (G + 1), c);
     const SnapMeta m = meta[s];
     if (c > m.rmax) return;
-    stage1_column<G, KR>(s, c, rg, sorted, m, tw, corr, kt, stride, I, fb, mass, sc, isc);
+    stage1_column<G, KR>(s, c, rg, sorted, m, tw, kt, stride, I, fb, mass, sc, isc);
 }
 
 // Persistent form: a fixed grid pulls (snapshot, column) items from a
@@ -599,7 +588,7 @@ __global__ __launch_bounds__(FftPlan<G>::nt, (FftPlan<G>::nt >= 1024 ? 1 : 2)) v
 template <int G, int KR>
 __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
     const VisRec* __restrict__ sorted, const int* __restrict__ bstart, const SnapMeta* __restrict__ meta,
-    const float2* __restrict__ tw, const float* __restrict__ corr, float2* __restrict__ kt, size_t stride,
+    const float2* __restrict__ tw, float2* __restrict__ kt, size_t stride,
     int I, int S, unsigned* work)
 {
     __shared__ float2 fb[lds_len(G)];
@@ -618,7 +607,7 @@ __global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
         const int4 rg = column_ranges<G>(bstart + (size_t)s * (G + 1), c);
         const SnapMeta m = meta[s];
         if (c > m.rmax) continue;
-        stage1_column<G, KR>(s, c, rg, sorted, m, tw, corr, kt, stride, I, fb, mass, sc, isc);
+        stage1_column<G, KR>(s, c, rg, sorted, m, tw, kt, stride, I, fb, mass, sc, isc);
         __syncthreads();
     }
 }
@@ -690,33 +679,47 @@ __device__ __forceinline__ const float2* c2r_rows(const float2* __restrict__ kt,
     return ks + (size_t)(y0 / kKTile) * (M + 1) * kKTile + (y0 % kKTile);
 }
 
-// d[2n] = Re z[n], d[2n+1] = Im z[n]; crop x in [-h, h), |.| and the column
-// correction (FIPkernels.cu:256-287) into one output row; returns the row
-// max of this thread.  Eight columns per thread per pass: the correction
-// loads of a pass are issued together.
-template <int M, int NT>
-__device__ __forceinline__ float c2r_output(const float2* z, const float* __restrict__ corr, float* out, int I, int t)
+// d[2n] = Re z[n], d[2n+1] = Im z[n]; crop x in [-h, h), |.| and the grid
+// correction 1/(c(|x|) c(|y|) norm^2) (FIPkernels.cu:256-287) for all ROWS
+// rows of a workgroup at once:
+// each thread takes whole columns, so a column's correction factor is loaded
+// once for all the rows; returns the thread's max.  out0: output row of the
+// workgroup's first row.
+template <int M, int ROWS, int NTB>
+__device__ __forceinline__ float c2r_output_rows(const float2* zb, const float* __restrict__ corr, float* out0,
+                                                 int y0, int I, int tid)
 {
     constexpr int G = 2 * M;
+    constexpr int L = lds_len(M + 1);
     const int h = I / 2;
     float lmax = 0.0f;
-    for (int c0 = t; c0 < I; c0 += 8 * NT) {
-        float cv[8];
+    // Row correction of each row (the grid correction is separable).
+    float cy[ROWS];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int x = c0 + u * NT - h;
-            cv[u] = (c0 + u * NT < I) ? corr[x < 0 ? -x : x] : 0.0f;
+    for (int r = 0; r < ROWS; ++r) {
+        const int y = y0 + r - h;
+        cy[r] = corr[y < 0 ? -y : y];
+    }
+    for (int c0 = tid; c0 < I; c0 += 4 * NTB) {
+        float cv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int x = c0 + u * NTB - h;
+            cv[u] = (c0 + u * NTB < I) ? corr[x < 0 ? -x : x] : 0.0f;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int col = c0 + u * NT;
+        for (int u = 0; u < 4; ++u) {
+            const int col = c0 + u * NTB;
             if (col < I) {
                 const int x = col - h;
                 const int mm = x < 0 ? x + G : x;
-                const float2 v = z[lds_pad(mm >> 1)];
-                const float val = fabsf((mm & 1) ? v.y : v.x) * cv[u];
-                out[col] = val;
-                lmax = fmaxf(lmax, val);
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r) {
+                    const float2 v = zb[r * L + lds_pad(mm >> 1)];
+                    const float val = fabsf((mm & 1) ? v.y : v.x) * (cv[u] * cy[r]);
+                    out0[(size_t)r * I + col] = val;
+                    lmax = fmaxf(lmax, val);
+                }
             }
         }
     }
@@ -748,7 +751,8 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
     __syncthreads();
     float2* z = zb + grp * L;
     fft_inverse<M>(z, tw, t);
-    const float lmax = c2r_output<M, NT>(z, corr, pre + (size_t)s * I * I + (size_t)(y0 + grp) * I, I, t);
+    const float lmax = c2r_output_rows<M, ROWS, NT * ROWS>(zb, corr, pre + (size_t)s * I * I + (size_t)y0 * I, y0,
+                                                          I, threadIdx.x);
     const unsigned bm = block_max_u<NT * ROWS>(__float_as_uint(lmax), shm);
     if (threadIdx.x == 0) atomicMax(max_slot(pmax, s, 0, (int)(blockIdx.x % kMaxSlots)), bm);
 }
@@ -1255,10 +1259,10 @@ static void stage1_dispatch(const WorkView& w, const PlanTables& p, int S, bool 
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stage1_persistent<G, KR>, NT, 0);
         const int nb = max(1, ncu * max(per_cu, 1));
         hipLaunchKernelGGL((k_stage1_persistent<G, KR>), dim3(nb), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
-                           p.tw_g, p.corr, w.kt, w.stride, p.I, S, w.work);
+                           p.tw_g, w.kt, w.stride, p.I, S, w.work);
     } else {
         hipLaunchKernelGGL((k_stage1<G, KR>), dim3((G / 2 + 1) * S), dim3(NT), 0, st, w.sorted, w.bstart, w.meta,
-                           p.tw_g, p.corr, w.kt, w.stride, p.I);
+                           p.tw_g, w.kt, w.stride, p.I);
     }
 }
 
