@@ -411,7 +411,8 @@ int fip_batch_device(const void* vis_re, const void* vis_im, int vis_type, const
         auto enqueue = [&]() -> int {
             if (int rc = run_imaging(*C, *P, C->work, in, S, nmax, cell_size, norm_dev, (int)norm_n, tu, st))
                 return rc;
-            return run_trigger(*C, *P, C->work, stride, bg.win_d.as<int>(), S - 2, (int)unit_size, d_tlisi, st);
+            // windows (k, k+1, k+2): the run form reads each image once per 8 windows
+            return run_trigger_run(*C, *P, C->work, stride, 0, S - 2, (int)unit_size, d_tlisi, st);
         };
         if (C->profiling) {
             // Eager launches with per-stage events (no graph).

@@ -279,6 +279,14 @@ int run_trigger(Context& C, const Plan& P, Work& W, size_t stride, const int* wi
     return FIP_OK;
 }
 
+int run_trigger_run(Context& C, const Plan& P, Work& W, size_t stride, int s0, int nwin, int tile, float* out_dev,
+                    hipStream_t st)
+{
+    const WorkView w = W.view(stride);
+    FIP_STAGE(C, kStTrigger, st, launch_trigger_run(w, P.t, s0, nwin, tile, out_dev, st));
+    return FIP_OK;
+}
+
 int check_shape(size_t num_baselines, size_t image_size, size_t unit_size, float cell_size)
 {
     if (num_baselines == 0) return set_error(FIP_ERR_ARG, "num_baselines must be positive");

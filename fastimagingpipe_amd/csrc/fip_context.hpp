@@ -92,6 +92,11 @@ int run_imaging(Context& C, const Plan& P, Work& W, const InputView& in, int S, 
 int run_trigger(Context& C, const Plan& P, Work& W, size_t stride, const int* win_dev, int nwin, int tile,
                 float* out_dev, hipStream_t st);
 
+// Enqueue the trigger over the consecutive windows (s0+k, s0+k+1, s0+k+2),
+// k < nwin.
+int run_trigger_run(Context& C, const Plan& P, Work& W, size_t stride, int s0, int nwin, int tile, float* out_dev,
+                    hipStream_t st);
+
 // Shape checks shared by every entry point.
 int check_shape(size_t num_baselines, size_t image_size, size_t unit_size, float cell_size);
 

@@ -151,6 +151,11 @@ hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tun
 hipError_t launch_trigger(const WorkView& w, const PlanTables& p, const int* win, int nwin,
                           int tile, float* out, hipStream_t st);
 
+// Trigger over the consecutive windows (s0+k, s0+k+1, s0+k+2), k < nwin,
+// reading each image once per group of windows.
+hipError_t launch_trigger_run(const WorkView& w, const PlanTables& p, int s0, int nwin, int tile, float* out,
+                              hipStream_t st);
+
 // meta[s].maxp_bits / maxd_bits from the spread slots, for host readers of
 // imaging results without a trigger launch.
 hipError_t launch_max_finalize(const WorkView& w, int S, hipStream_t st);

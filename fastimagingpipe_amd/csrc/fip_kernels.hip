@@ -1192,6 +1192,87 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
     }
 }
 
+// Consecutive windows (s0+w, s0+w+1, s0+w+2), w < nwin, of a batch: one wave
+// per tile handles up to kRunWin consecutive windows, so each image is read
+// once per kRunWin windows instead of three times per window.
+constexpr int kRunWin = 8;
+
+__global__ __launch_bounds__(256) void k_trigger_run(const float* __restrict__ img, SnapMeta* __restrict__ meta,
+                                                     unsigned* __restrict__ pmax, int s0, int nwin, int I, int T,
+                                                     float* __restrict__ out)
+{
+    constexpr int K = kRunWin, NI = kRunWin + 2;
+    const int un = I / T;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= un * un) return;   // whole wave
+    const int w0 = blockIdx.y * K, nk = min(K, nwin - w0), ni = nk + 2;
+    const int i0 = s0 + w0;
+    const int ti = tile / un, tj = tile % un;
+    // Image maxima from the spread slots (one slot per lane).
+    unsigned md[NI];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) md[j] = wave_max_u(j < ni ? *max_slot(pmax, i0 + j, 1, lane) : 0u);
+    const float C = 1e-6f;   // FIPkernels.cu:811
+    float sum[K], mx[K], sr[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) sum[k] = mx[k] = sr[k] = 0.0f;
+    auto px = [&](int k, float a, float b, float c) {
+        const float dd = fabsf(fabsf(a - b) - fabsf(b - c));
+        const float b2 = (b <= 0.0f) ? C : b;
+        sum[k] += dd;
+        mx[k] = fmaxf(mx[k], dd);
+        const float ratio = dd / b2;
+        sr[k] += (ratio < 1.0f) ? ratio : 1.0f;
+    };
+    const size_t plane = (size_t)I * I;
+    const float* base = img + (size_t)i0 * plane;
+    if ((T & 3) == 0) {
+        const int T4 = T >> 2, n4 = T * T4;
+        for (int p = lane; p < n4; p += 64) {
+            const size_t q = (size_t)(ti * T + p / T4) * I + (size_t)(tj * T + 4 * (p % T4));
+            float4 d[NI];
+#pragma unroll
+            for (int j = 0; j < NI; ++j)
+                d[j] = j < ni ? *reinterpret_cast<const float4*>(base + (size_t)j * plane + q)
+                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (k < nk) {
+                    px(k, d[k].x, d[k + 1].x, d[k + 2].x);
+                    px(k, d[k].y, d[k + 1].y, d[k + 2].y);
+                    px(k, d[k].z, d[k + 1].z, d[k + 2].z);
+                    px(k, d[k].w, d[k + 1].w, d[k + 2].w);
+                }
+            }
+        }
+    } else {
+        for (int p = lane; p < T * T; p += 64) {
+            const size_t q = (size_t)(ti * T + p / T) * I + (size_t)(tj * T + p % T);
+            float d[NI];
+#pragma unroll
+            for (int j = 0; j < NI; ++j) d[j] = j < ni ? base[(size_t)j * plane + q] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k < nk) px(k, d[k], d[k + 1], d[k + 2]);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the reduced maxima for host readers
+        for (int j = 0; j < ni; ++j) meta[i0 + j].maxd_bits = md[j];
+    }
+    const float Tf = (float)T;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float sk = wave_sum_f(sum[k]), mk = wave_max_f(mx[k]), rk = wave_sum_f(sr[k]);
+        if (lane == 0 && k < nk) {
+            const float m3 = fmaxf(__uint_as_float(md[k]), fmaxf(__uint_as_float(md[k + 1]), __uint_as_float(md[k + 2])));
+            // FIPkernels.cu:550
+            out[(size_t)(w0 + k) * un * un + tile] = 1.0f - (sk / Tf / Tf) * mk * (rk / Tf / Tf) / m3 / m3;
+        }
+    }
+}
+
 // ------------------------------------------------------------- launchers
 
 const char* stage_name(int st)
@@ -1357,6 +1438,16 @@ hipError_t launch_trigger(const WorkView& w, const PlanTables& p, const int* win
     if (un <= 0 || nwin <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_trigger, dim3((un * un + 3) / 4, nwin), dim3(256), 0, st, w.img, w.meta, w.pmax, win, p.I,
                        tile, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_trigger_run(const WorkView& w, const PlanTables& p, int s0, int nwin, int tile, float* out,
+                              hipStream_t st)
+{
+    const int un = p.I / tile;
+    if (un <= 0 || nwin <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trigger_run, dim3((un * un + 3) / 4, (nwin + kRunWin - 1) / kRunWin), dim3(256), 0, st,
+                       w.img, w.meta, w.pmax, s0, nwin, p.I, tile, out);
     return hipGetLastError();
 }
 
