@@ -50,7 +50,7 @@ __host__ __device__ inline unsigned* max_slot(unsigned* pm, int s, int set, int 
 struct SnapMeta {
     int ngrid;            // visibilities with at least one grid tap
     int rmax;             // largest |column tap|: stage 1 covers c = 0..rmax
-    int r1max;            // largest |row tap|: the int64 gridding band
+    int r1max;            // largest |row tap|: rows of the gridded band
     unsigned maxv_bits;   // max(|re|,|im|) of weighted visibilities (float bits)
     unsigned maxp_bits;   // max of the corrected image before re-projection (k_max_finalize)
     unsigned maxd_bits;   // max of the re-projected image, maxall entry (k_trigger, k_max_finalize)
@@ -104,7 +104,7 @@ struct WorkView {
     int* bstart;       // [S*(G+1)]
     SnapMeta* meta;    // [S]
     SnapGeom* geom;    // [S]
-    float2* kt;        // [S][I/16][M+1][16]
+    float2* kt;        // [S][I/kKTile][M+1][kKTile]
     float* pre;        // [S][I][I] corrected image before re-projection
     float* img;        // [S][I][I] re-projected images
     unsigned* work;    // [1] persistent-kernel work counter

@@ -636,10 +636,28 @@ struct C2RGather {
             const int p = tid + u * NT;
             const bool ok = p <= M / 2;
             wv[u] = ok ? wpost[p] : make_float2(0.0f, 0.0f);
+            if constexpr (ROWS % 2 == 0) {
+                // Two rows per 16-byte load: y0 is a multiple of ROWS, so the
+                // pair is 16-byte aligned inside its 64-byte tile run.
 #pragma unroll
-            for (int r = 0; r < ROWS; ++r) {
-                a[u][r] = (ok && p <= cmax) ? kt0[(size_t)p * kKTile + r] : make_float2(0.0f, 0.0f);
-                b[u][r] = (ok && M - p <= cmax) ? kt0[(size_t)(M - p) * kKTile + r] : make_float2(0.0f, 0.0f);
+                for (int r = 0; r < ROWS; r += 2) {
+                    const float4 va = (ok && p <= cmax)
+                                          ? *reinterpret_cast<const float4*>(kt0 + (size_t)p * kKTile + r)
+                                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    const float4 vb = (ok && M - p <= cmax)
+                                          ? *reinterpret_cast<const float4*>(kt0 + (size_t)(M - p) * kKTile + r)
+                                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    a[u][r] = make_float2(va.x, va.y);
+                    a[u][r + 1] = make_float2(va.z, va.w);
+                    b[u][r] = make_float2(vb.x, vb.y);
+                    b[u][r + 1] = make_float2(vb.z, vb.w);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r) {
+                    a[u][r] = (ok && p <= cmax) ? kt0[(size_t)p * kKTile + r] : make_float2(0.0f, 0.0f);
+                    b[u][r] = (ok && M - p <= cmax) ? kt0[(size_t)(M - p) * kKTile + r] : make_float2(0.0f, 0.0f);
+                }
             }
         }
     }

@@ -77,8 +77,10 @@ int fip_pipe_device(const float *vis_re, const float *vis_im, const float *bin,
 /*
  * Precision form of fip_pipe: visibilities stored as bfloat16 or IEEE half
  * (vis_type FIP_VIS_BF16 / FIP_VIS_F16; 2-byte elements), converted to FP32
- * on load; gridding accumulates scaled, integer-rounded contributions in
- * FP64 (exact, so independent of summation order) and the FFTs run in FP32 (FIPkernels.cu:188-245 for the arithmetic it replaces).
+ * on load; gridding accumulates contributions scaled by a power of two per
+ * block of rows and rounded to integers in int32 (exact, so independent of
+ * summation order) and the FFTs run in FP32 (FIPkernels.cu:188-245 for the
+ * arithmetic it replaces).
  */
 int fip_pipe_lowp(const void *vis_re, const void *vis_im, int vis_type,
                   const float *bin, const float *vin, float *result,
