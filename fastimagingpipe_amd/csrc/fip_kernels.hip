@@ -212,25 +212,30 @@ __global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ n
 
 // ------------------------------------------------------------------- bin
 
-// Visibilities per thread of the bin kernel (a block bins 256 * kBinVpt).
-constexpr int kBinVpt = 8;
+// Threads of a bin workgroup and visibilities per thread (a workgroup bins
+// kBinThreads * kBinVpt visibilities).  Large workgroups keep the number of
+// per-workgroup bucket flushes (global atomics) low; 16 waves each keep the
+// CUs busy although a step has only ~100 workgroups.
+constexpr int kBinThreads = 1024;
+constexpr int kBinVpt = 4;
 
 // Block-privatised column histogram: slots within the block come from LDS
 // atomics, then one global atomic per non-empty bucket reserves the block's
 // range, so the hot central buckets see one global atomic per block instead
 // of one per visibility.
 template <class VT>
-__global__ __launch_bounds__(256) void k_bin(InputView in, VisRec* __restrict__ rec, int2* __restrict__ bs,
+__global__ __launch_bounds__(kBinThreads) void k_bin(InputView in, VisRec* __restrict__ rec, int2* __restrict__ bs,
                                              int* __restrict__ hist, SnapMeta* __restrict__ meta, int G,
                                              double scale)
 {
     extern __shared__ int lh[];   // G ints: local counts, then block bases
-    __shared__ int sh_i[3][4];
-    __shared__ unsigned sh_u[4];
+    constexpr int NW = kBinThreads / 64;
+    __shared__ int sh_i[3][NW];
+    __shared__ unsigned sh_u[NW];
     const int s = blockIdx.y;
     const int ns = in.nvis ? in.nvis[s] : in.n_const;
     const int M = G / 2;
-    for (int b = threadIdx.x; b < G; b += 256) lh[b] = 0;
+    for (int b = threadIdx.x; b < G; b += kBinThreads) lh[b] = 0;
     __syncthreads();
     int rmax = -1, r1max = -1, cnt = 0;
     unsigned vmax = 0u;
@@ -242,7 +247,7 @@ __global__ __launch_bounds__(256) void k_bin(InputView in, VisRec* __restrict__ 
     for (int k = 0; k < kBinVpt; ++k) {
         bk[k] = -1;
         slot[k] = 0;
-        const int n = blockIdx.x * (256 * kBinVpt) + k * 256 + threadIdx.x;
+        const int n = blockIdx.x * (kBinThreads * kBinVpt) + k * kBinThreads + threadIdx.x;
         if (n >= ns) continue;
         const size_t q = (size_t)s * in.stride + (size_t)n;
         const float vr = (float)((double)vis_to_f<VT>(in.vis_re, q) / J);
@@ -272,14 +277,14 @@ __global__ __launch_bounds__(256) void k_bin(InputView in, VisRec* __restrict__ 
         vmax = max(vmax, __float_as_uint(fmaxf(fabsf(vr), fabsf(vi))));
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < G; b += 256) {
+    for (int b = threadIdx.x; b < G; b += kBinThreads) {
         const int c = lh[b];
         if (c > 0) lh[b] = atomicAdd(&hist[(size_t)s * G + b], c);
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kBinVpt; ++k) {
-        const int n = blockIdx.x * (256 * kBinVpt) + k * 256 + threadIdx.x;
+        const int n = blockIdx.x * (kBinThreads * kBinVpt) + k * kBinThreads + threadIdx.x;
         if (n >= ns) continue;
         const size_t q = (size_t)s * in.stride + (size_t)n;
         bs[q] = bk[k] >= 0 ? make_int2(bk[k], lh[bk[k]] + slot[k]) : make_int2(-1, 0);
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(256) void k_bin(InputView in, VisRec* __restrict__ 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; ++k) {
+        for (int k = 1; k < NW; ++k) {
             rmax = max(rmax, sh_i[0][k]);
             r1max = max(r1max, sh_i[1][k]);
             vmax = max(vmax, sh_u[k]);
@@ -1313,18 +1318,18 @@ hipError_t launch_bin(const InputView& in, const WorkView& w, const PlanTables& 
                       double scale, hipStream_t st)
 {
     if (nmax <= 0) return hipSuccess;
-    dim3 grid((nmax + 256 * kBinVpt - 1) / (256 * kBinVpt), S);
+    dim3 grid((nmax + kBinThreads * kBinVpt - 1) / (kBinThreads * kBinVpt), S);
     const size_t lds = (size_t)p.G * sizeof(int);
     switch (in.vis_type) {
     case kVisF32:
-        hipLaunchKernelGGL(k_bin<float>, grid, dim3(256), lds, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G, scale);
+        hipLaunchKernelGGL(k_bin<float>, grid, dim3(kBinThreads), lds, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G, scale);
         break;
     case kVisBF16:
-        hipLaunchKernelGGL(k_bin<__hip_bfloat16>, grid, dim3(256), lds, st, in, w.rec, w.bkslot, w.hist, w.meta,
+        hipLaunchKernelGGL(k_bin<__hip_bfloat16>, grid, dim3(kBinThreads), lds, st, in, w.rec, w.bkslot, w.hist, w.meta,
                            p.G, scale);
         break;
     case kVisF16:
-        hipLaunchKernelGGL(k_bin<__half>, grid, dim3(256), lds, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G,
+        hipLaunchKernelGGL(k_bin<__half>, grid, dim3(kBinThreads), lds, st, in, w.rec, w.bkslot, w.hist, w.meta, p.G,
                            scale);
         break;
     default:
