@@ -203,10 +203,11 @@ Tuning Context::tuning(size_t image) const
     if (it != tunings_.end()) return it->second;
     Tuning t;
     t.stage1_records = 4;         // gridding records per thread in registers
-    // Rows per C2R workgroup, so that a workgroup holds about 50 KB of LDS
-    // and three share a CU: four rows of up to 1536 points, two of 3072, one
-    // of 6144 (measured 10-25% faster than two rows at 1024, 2048 and 8192).
-    t.stage2_rows = image <= 2048 ? 4 : (image <= 4096 ? 2 : 1);
+    // Rows per C2R workgroup: four rows of up to 768 points, two of 1536 and
+    // 3072 (each pair of rows read with one 16-byte load per K entry: 6%
+    // faster than four rows at 2048), one of 6144 (two rows of it hold
+    // 100 KB of LDS, one workgroup per CU).
+    t.stage2_rows = image <= 1024 ? 4 : (image <= 4096 ? 2 : 1);
     t.splat_tile = 64;
     t.persistent = 0;
     return t;
