@@ -319,11 +319,15 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(InputView in, VisRec* __res
 
 // ------------------------------------------------------------------ scan
 
+// Largest grid size: bounds the bucket counts one scan thread holds.
+constexpr int kMaxGrid = 12288;
+
 __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ hist, int* __restrict__ bstart, int G)
 {
-    // Exclusive prefix of the bucket counts: each thread sums a contiguous
-    // chunk, waves scan their chunk sums with shuffles, one wave scans the
-    // 16 wave totals.
+    // Exclusive prefix of the bucket counts: each thread loads a contiguous
+    // chunk into registers (one round of loads), waves scan their chunk sums
+    // with shuffles, one wave scans the 16 wave totals.
+    constexpr int PER = (kMaxGrid + 1023) / 1024;
     __shared__ int wsum[16];
     const int s = blockIdx.x;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
@@ -331,8 +335,12 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ hist, int
     const int beg = min(t * per, G), end = min(beg + per, G);
     const int* h = hist + (size_t)s * G;
     int* o = bstart + (size_t)s * (G + 1);
+    int v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) v[k] = (beg + k < end) ? h[beg + k] : 0;
     int local = 0;
-    for (int i = beg; i < end; ++i) local += h[i];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) local += v[k];
     int x = local;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -352,9 +360,10 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ hist, int
     }
     __syncthreads();
     int run = x - local + (wid > 0 ? wsum[wid - 1] : 0);
-    for (int i = beg; i < end; ++i) {
-        o[i] = run;
-        run += h[i];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (beg + k < end) o[beg + k] = run;
+        run += v[k];
     }
     if (t == 1023) o[G] = wsum[15];
 }
@@ -1340,6 +1349,7 @@ hipError_t launch_bin(const InputView& in, const WorkView& w, const PlanTables& 
 
 hipError_t launch_scan(const WorkView& w, const PlanTables& p, int S, hipStream_t st)
 {
+    if (p.G > kMaxGrid) return hipErrorInvalidValue;   // a scan thread holds at most 12 counts
     hipLaunchKernelGGL(k_scan, dim3(S), dim3(1024), 0, st, w.hist, w.bstart, p.G);
     return hipGetLastError();
 }
