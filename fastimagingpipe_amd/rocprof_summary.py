@@ -12,6 +12,11 @@
         what it reads for the device-to-device copy kernel of N bytes that
         the profiled program runs, so a plain copy of N bytes counts N bytes
         read and N bytes written.
+
+    python -m fastimagingpipe_amd.rocprof_summary counters DIR [DIR ...]
+        Average per dispatch of every counter in the ``rocprofv3 --pmc`` runs
+        under the DIRs (one pass each), per pipeline stage, and ratios of
+        those counters as the hardware reports them.
 """
 from __future__ import annotations
 
@@ -89,6 +94,33 @@ def counter_values(directory: str, counter: str) -> dict[str, list[float]]:
     return {k: list(v.values()) for k, v in vals.items()}
 
 
+def counter_table(directories: list[str]) -> dict[str, dict[str, float]]:
+    """Per stage: average per dispatch of every counter found under the directories."""
+    sums = defaultdict(lambda: defaultdict(dict))   # stage -> counter -> dispatch -> value
+    for d in directories:
+        for path in _find(d, "counter_collection.csv"):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    st = stage_of(_col(r, "Kernel_Name"))
+                    if st is None:
+                        continue
+                    cn = _col(r, "Counter_Name")
+                    key = (d, _col(r, "Dispatch_Id", "Correlation_Id"))
+                    sums[st][cn][key] = sums[st][cn].get(key, 0.0) + float(_col(r, "Counter_Value"))
+    return {st: {cn: sum(v.values()) / len(v) for cn, v in cs.items()} for st, cs in sums.items()}
+
+
+# Derived ratios: name -> (numerator, denominator).
+RATIOS = [
+    ("VALU issue / wave cycles", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES"),
+    ("LDS issue / wave cycles", "SQ_ACTIVE_INST_LDS", "SQ_WAVE_CYCLES"),
+    ("LDS wait / wave cycles", "SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES"),
+    ("any wait / wave cycles", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES"),
+    ("LDS bank-conflict / LDS active", "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"),
+    ("LDS instr / VALU instr", "SQ_INSTS_LDS", "SQ_INSTS_VALU"),
+]
+
+
 def traffic(fetch_dir: str, write_dir: str, copy_bytes: int) -> dict:
     """copy_bytes: size of ONE calibration copy; the largest copy dispatch is taken
     as that copy (smaller copy dispatches are the program's input uploads)."""
@@ -132,7 +164,23 @@ def main(argv=None) -> int:
     b.add_argument("write_dir")
     b.add_argument("--copy-bytes", type=int, required=True)
     b.add_argument("--json")
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
     args = ap.parse_args(argv)
+    if args.cmd == "counters":
+        tab = counter_table(args.dirs)
+        order = [st for _, st in STAGES if st in tab]
+        order = list(dict.fromkeys(order))
+        names = sorted({cn for st in tab for cn in tab[st]})
+        print(f"{'counter (avg per dispatch)':<32}" + "".join(f"{st:>14}" for st in order))
+        for cn in names:
+            print(f"{cn:<32}" + "".join(f"{tab[st].get(cn, float('nan')):>14.4g}" for st in order))
+        print()
+        for label, num, den in RATIOS:
+            if all(num in tab[st] and den in tab[st] for st in order):
+                print(f"{label:<32}" + "".join(
+                    f"{(tab[st][num] / tab[st][den] if tab[st][den] else float('nan')):>14.3f}" for st in order))
+        return 0
     if args.cmd == "stats":
         rows = kernel_stats(args.dir)
         print(f"{'kernel':<72} {'stage':<12} {'calls':>6} {'avg_us':>10} {'total_ms':>10} {'pct':>6}")
