@@ -376,8 +376,10 @@ __global__ __launch_bounds__(256) void k_scatter(const VisRec* __restrict__ rec,
 {
     const int s = blockIdx.y;
     const int n = blockIdx.x * 256 + threadIdx.x;
-    // n < max count <= stride, so the loads are in bounds before the count
-    // of this snapshot is known; all three are in flight together.
+    // The last workgroup reaches past the largest count; n < stride keeps
+    // the loads inside this snapshot's rows, so they are issued before its
+    // count is known and all three are in flight together.
+    if ((size_t)n >= stride) return;
     const size_t q = (size_t)s * stride + (size_t)n;
     const int2 key = bs[q];
     const VisRec r = rec[q];
