@@ -1147,6 +1147,16 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
 
 // --------------------------------------------------------------- trigger
 
+// Non-temporal 16-byte load for the trigger, which reads each image byte
+// once.  Measured on the one-window trigger: 157 -> 128 us at 8192^2 (three
+// images of 268 MB), unchanged at 4096^2; same bits.
+typedef float fip_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_stream(const float* p)
+{
+    const fip_f4 v = __builtin_nontemporal_load(reinterpret_cast<const fip_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // One wave per tile (four tiles per workgroup): per-lane partial sums over
 // 16-byte loads, then wave reductions only (FIPkernels.cu:499-552).
 __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, SnapMeta* __restrict__ meta,
@@ -1187,9 +1197,9 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
                 const int p = p0 + u * 64;
                 if (p < n4) {
                     const size_t q = (size_t)(ti * T + p / T4) * I + (size_t)(tj * T + 4 * (p % T4));
-                    a[u] = *reinterpret_cast<const float4*>(d1 + q);
-                    b[u] = *reinterpret_cast<const float4*>(d2 + q);
-                    c[u] = *reinterpret_cast<const float4*>(d3 + q);
+                    a[u] = load_stream(d1 + q);
+                    b[u] = load_stream(d2 + q);
+                    c[u] = load_stream(d3 + q);
                 }
             }
 #pragma unroll
@@ -1268,7 +1278,7 @@ __global__ __launch_bounds__(256) void k_trigger_run(const float* __restrict__ i
             float4 d[NI];
 #pragma unroll
             for (int j = 0; j < NI; ++j)
-                d[j] = j < ni ? *reinterpret_cast<const float4*>(base + (size_t)j * plane + q)
+                d[j] = j < ni ? load_stream(base + (size_t)j * plane + q)
                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
