@@ -751,7 +751,9 @@ __device__ __forceinline__ float c2r_output_rows(const float2* zb, const float* 
                 for (int r = 0; r < ROWS; ++r) {
                     const float2 v = zb[r * L + lds_pad(mm >> 1)];
                     const float val = fabsf((mm & 1) ? v.y : v.x) * (cv[u] * cy[r]);
-                    out0[(size_t)r * I + col] = val;
+                    // Non-temporal store: 122-127 -> 108-109 us at 4096^2, and
+                    // the splat that reads these images next is not slower.
+                    __builtin_nontemporal_store(val, out0 + (size_t)r * I + col);
                     lmax = fmaxf(lmax, val);
                 }
             }
