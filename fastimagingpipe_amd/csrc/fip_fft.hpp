@@ -244,13 +244,15 @@ template <int Ns, int R> struct StoreOff {
 };
 
 // One Stockham stage of radix R with Ns = product of earlier radices.
-// tw points at this stage's twiddles laid out [r-1][k], k < Ns.
+// tw points at this stage's twiddles laid out [r-1][k], k < Ns.  Threads
+// t >= NT of a larger workgroup take no butterfly but reach both barriers.
 template <int N, int NT, int R, int Ns>
 __device__ __forceinline__ void fft_stage(float2* __restrict__ buf, const float2* __restrict__ tw, int t)
 {
     constexpr int NB = N / R;                    // butterflies in this stage
     constexpr int BPT = (NB + NT - 1) / NT;      // butterflies per thread
     constexpr bool exact = (NB % NT) == 0;
+    const bool act = t < NT;
     float2 v[BPT][R];
     // Twiddles do not depend on the data: issue their (L1/L2-resident) loads
     // before the LDS loads and the barrier so their latency is hidden.
@@ -267,7 +269,7 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ buf, const float2
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
         const int j = t + b * NT;
-        if (exact || j < NB) {
+        if (act && (exact || j < NB)) {
             if constexpr (LoadOff<NB>::ok) {
                 const float2* p = buf + lds_pad(j);
 #pragma unroll
@@ -282,7 +284,7 @@ __device__ __forceinline__ void fft_stage(float2* __restrict__ buf, const float2
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
         const int j = t + b * NT;
-        if (exact || j < NB) {
+        if (act && (exact || j < NB)) {
             const int k = j % Ns;
             if (Ns > 1) {
 #pragma unroll

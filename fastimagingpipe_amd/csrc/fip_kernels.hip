@@ -472,6 +472,15 @@ __device__ __forceinline__ int4 column_ranges(const int* __restrict__ bst, int c
     return r;
 }
 
+// Threads of a stage-1 workgroup: the column FFT's own count, but at least
+// 256, so that a column with many records still spreads its gridding over
+// four waves (the extra threads idle through the FFT stages).
+template <int G>
+constexpr int stage1_threads()
+{
+    return FftPlan<G>::nt < 256 ? 256 : FftPlan<G>::nt;
+}
+
 template <int G, int KR>
 __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRec* __restrict__ sorted,
                                               const SnapMeta& m,
@@ -479,7 +488,7 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
                                               float2* __restrict__ kt, size_t stride, int I, float2* fb,
                                               unsigned* mass, float* sc, float* isc)
 {
-    constexpr int NT = FftPlan<G>::nt;
+    constexpr int NT = stage1_threads<G>();
     constexpr int M = G / 2;
     const int t = threadIdx.x;
 
@@ -577,7 +586,7 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
 }
 
 template <int G, int KR>
-__global__ __launch_bounds__(FftPlan<G>::nt, (FftPlan<G>::nt >= 1024 ? 1 : 2)) void k_stage1(const VisRec* __restrict__ sorted,
+__global__ __launch_bounds__(stage1_threads<G>(), (stage1_threads<G>() >= 1024 ? 1 : 2)) void k_stage1(const VisRec* __restrict__ sorted,
                                                             const int* __restrict__ bstart,
                                                             const SnapMeta* __restrict__ meta,
                                                             const float2* __restrict__ tw,
@@ -602,7 +611,7 @@ __global__ __launch_bounds__(FftPlan<G>::nt, (FftPlan<G>::nt >= 1024 ? 1 : 2)) v
 // Persistent form: a fixed grid pulls (snapshot, column) items from a
 // counter until all S*(M+1) items are claimed; every workgroup exits.
 template <int G, int KR>
-__global__ __launch_bounds__(FftPlan<G>::nt) void k_stage1_persistent(
+__global__ __launch_bounds__(stage1_threads<G>()) void k_stage1_persistent(
     const VisRec* __restrict__ sorted, const int* __restrict__ bstart, const SnapMeta* __restrict__ meta,
     const float2* __restrict__ tw, float2* __restrict__ kt, size_t stride,
     int I, int S, unsigned* work)
@@ -1379,7 +1388,7 @@ hipError_t launch_scatter(const WorkView& w, const PlanTables& p, int S, int nma
 template <int G, int KR>
 static void stage1_dispatch(const WorkView& w, const PlanTables& p, int S, bool persistent, hipStream_t st)
 {
-    constexpr int NT = FftPlan<G>::nt;
+    constexpr int NT = stage1_threads<G>();
     if (persistent) {
         int dev = 0, ncu = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
