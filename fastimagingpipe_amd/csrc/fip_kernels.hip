@@ -472,13 +472,14 @@ __device__ __forceinline__ int4 column_ranges(const int* __restrict__ bst, int c
     return r;
 }
 
-// Threads of a stage-1 workgroup: the column FFT's own count, but at least
-// 256, so that a column with many records still spreads its gridding over
-// four waves (the extra threads idle through the FFT stages).
+// Threads of a stage-1 workgroup: the column FFT's own count, or 512 where
+// that count is below 256 (g <= 1536), so that a column with many records
+// still spreads its gridding over eight waves; the extra threads idle
+// through the FFT stages.  (A floor of 512 for g = 3072 measured slower.)
 template <int G>
 constexpr int stage1_threads()
 {
-    return FftPlan<G>::nt < 256 ? 256 : FftPlan<G>::nt;
+    return FftPlan<G>::nt < 256 ? 512 : FftPlan<G>::nt;
 }
 
 template <int G, int KR>
