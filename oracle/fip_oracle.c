@@ -15,8 +15,16 @@
  *       answer |Re(v e^{2 pi i (p1 y + p2 x)/g})| / J / c / c * (ES(.)...) and
  *       a brute-force direct tap-DFT of the whole chain; linearity.
  *   fo_coordschange + fo_p2p : V = identity gives the column mirror
- *       (2h-i, j); against the closed SIN form P1 - xi n'/dc, P2 + eta n'/dc.
- *   fo_splat : integer targets copy exactly; mass conservation.
+ *       (2h-i, j); against the closed SIN form P1 - xi n'/dc, P2 + eta n'/dc;
+ *       a point source of a flat array seen in random tilted frames lands at
+ *       row sign(V22) l/cell + h, column -sign(V22) m/cell + h (sky geometry
+ *       and V orthonormal; mutation tests plant a swapped V01/V10, a dropped
+ *       or signed |V22| divisor and check that this pin fails).
+ *   fo_splat : integer targets copy exactly; mass conservation; a source at
+ *       fractional (LL, MM) keeps its centroid and gives a rank-one 2 x 2
+ *       footprint (a swapped {LL}/{MM} fails it).
+ *   fo_project (end to end): a source placed on pre-image pixel (y, x) of
+ *       the PCA frame images to exactly N/|V22| there (un-centred B).
  *   fo_max, fo_trigger : constant images give tLISI = 1; a single changed
  *       pixel gives the hand-computed tile value; numpy brute force.
  *   fo_project : against scikit-learn PCA fixtures (tests/golden/).

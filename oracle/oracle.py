@@ -13,7 +13,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libfip_oracle.so")
+# FIP_ORACLE_LIB names another build of fip_oracle.c (the mutation tests load
+# deliberately broken copies through it); by default the in-tree library.
+_LIB_OVERRIDE = os.environ.get("FIP_ORACLE_LIB")
+_LIB_PATH = _LIB_OVERRIDE or os.path.join(_HERE, "libfip_oracle.so")
 _lib = None
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -24,6 +27,8 @@ _sz = ctypes.c_size_t
 
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if _LIB_OVERRIDE:
+        return _LIB_PATH
     if force or not os.path.exists(_LIB_PATH):
         subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []),
                        check=True)

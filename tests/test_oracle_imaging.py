@@ -232,3 +232,180 @@ def test_collapse_special_cases(oracle):
     assert out[2] == pytest.approx(yy[2])                     # XX flagged: YY
     assert out[3] == 0                                        # all flagged: 0
     assert oracle.collapse([1j], [2j], [0], [0], [0.0], [0.0])[0] == 0   # zero weights
+
+
+# ----------------------------------------------------------------------------
+# Geometry known answers: where a point source of the sky lands.
+#
+# A flat array seen from a tilted uvw frame has coplanar baselines D, so the
+# PCA normal V2 = V0 x V1 is the plane's normal and D . V2 = 0.  A source at
+# direction cosines (l, m) has visibilities exp(-2 pi i D . s'), s' =
+# (l, m, n - 1), and D . s' = B0 (V0 . s') + B1 (V1 . s').  The gridding
+# phase e^{+2 pi i (B0 cell y + B1 cell x)} therefore peaks at row
+# y = V0 . s' / cell, column x = V1 . s' / cell of the pre-image, whatever V.
+# Re-projection undoes V: with V orthonormal, the cofactors of V22 give
+# -V00 (V1 . s') + V10 (V0 . s') = -V22 m + V21 (n - 1) and
+# -V01 (V1 . s') + V11 (V0 . s') = V22 l - V20 (n - 1), so after the division
+# by |V22| the source sits at column -sign(V22) m / cell + h and row
+# sign(V22) l / cell + h, up to the w terms (n - 1)/cell, which are below
+# 0.2 px for the fields used here.  A V with V22 < 0 (the PCA's sign rule
+# decides it) gives the point-mirrored sky.
+
+C_LIGHT = 299792458.0
+
+
+def _coplanar_array(rng, nant, cell, tilt):
+    """Baselines (wavelengths) of a flat array in a plane tilted by `tilt`."""
+    # antenna radius: longest baseline |B| cell <= 0.5, inside the half grid
+    rad = 0.25 / cell
+    r = rad * np.sqrt(rng.uniform(0, 1, nant))
+    a = rng.uniform(0, 2 * np.pi, nant)
+    pos = np.stack([r * np.cos(a), r * np.sin(a), np.zeros(nant)], axis=1)
+    i, j = np.triu_indices(nant, 1)
+    R = rotation(tilt, rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi))
+    return (pos[i] - pos[j]) @ R
+
+
+def _peak(img):
+    """Sub-pixel peak: argmax refined by the centroid of its 3x3 neighbourhood."""
+    r, c = np.unravel_index(np.argmax(img), img.shape)
+    w = img[r - 1:r + 2, c - 1:c + 2]
+    yy, xx = np.mgrid[-1:2, -1:2]
+    return r + (w * yy).sum() / w.sum(), c + (w * xx).sum() / w.sum()
+
+
+GEOM_SEEDS = [11, 12, 13, 14, 15, 16, 17, 18]
+
+
+def _geometry_case(oracle, seed, image=256, cell=5.8e-5, nant=64):
+    rng = np.random.default_rng(seed)
+    tilt = rng.uniform(0.35, 0.5)
+    uvw = _coplanar_array(rng, nant, cell, tilt)
+    # 60..100 px from the centre: a 1/|V22| error moves it by >= 3 px
+    rad = rng.uniform(60, 100) * cell
+    ang = rng.uniform(0, 2 * np.pi)
+    l, m = rad * np.cos(ang), rad * np.sin(ang)
+    n = np.sqrt(1 - l * l - m * m)
+    vis = np.exp(-2j * np.pi * (uvw[:, 0] * l + uvw[:, 1] * m + uvw[:, 2] * (n - 1)))
+    B, V = oracle.project(uvw, np.array([C_LIGHT]))   # D = uvw (already wavelengths)
+    pre, dirty, _ = oracle.image(vis.real.astype(np.float32), vis.imag.astype(np.float32),
+                                 B.astype(np.float32), V.astype(np.float32), image, cell,
+                                 want_pre=True)
+    return dict(V=V, l=l, m=m, n=n, pre=pre, dirty=dirty, tilt=tilt)
+
+
+def _source_on_pixel(V, y, x, cell):
+    """(l, m, n) with V0 . s' = y cell and V1 . s' = x cell, s' = (l, m, n - 1)."""
+    l = m = 0.0
+    for _ in range(50):
+        n1 = np.sqrt(1 - l * l - m * m) - 1
+        A = V[:2, :2]
+        rhs = np.array([y * cell - V[0, 2] * n1, x * cell - V[1, 2] * n1])
+        l, m = np.linalg.solve(A, rhs)
+    return l, m, np.sqrt(1 - l * l - m * m)
+
+
+@pytest.mark.parametrize("seed", GEOM_SEEDS[:4])
+def test_point_source_on_pixel_known_amplitude(oracle, seed):
+    """A unit source placed so that its PCA-frame position falls on pre-image
+    pixel (y, x): every baseline's phase there is exactly zero (D . V2 = 0),
+    so the corrected pre-image holds N / J, J = |V00 V11 - V01 V10| = |V22|,
+    at that pixel (FIPkernels.cu:188-287; B un-centred, SVD_MFS.py:105-110)."""
+    image, cell, nant = 256, 5.8e-5, 64
+    h = image // 2
+    rng = np.random.default_rng(seed)
+    uvw = _coplanar_array(rng, nant, cell, rng.uniform(0.35, 0.5))
+    B, V = oracle.project(uvw, np.array([C_LIGHT]))
+    y, x = (int(v) for v in rng.integers(-90, 90, 2))
+    l, m, n = _source_on_pixel(V, y, x, cell)
+    vis = np.exp(-2j * np.pi * (uvw[:, 0] * l + uvw[:, 1] * m + uvw[:, 2] * (n - 1)))
+    pre, _, _ = oracle.image(vis.real.astype(np.float32), vis.imag.astype(np.float32),
+                             B.astype(np.float32), V.astype(np.float32), image, cell, want_pre=True)
+    N = uvw.shape[0]
+    J = abs(V[0, 0] * V[1, 1] - V[0, 1] * V[1, 0])
+    assert J == pytest.approx(abs(V[2, 2]), rel=1e-12)
+    assert np.unravel_index(np.argmax(pre), pre.shape) == (y + h, x + h)
+    # float32 inputs (B, vis) and the gridding kernel's correction error
+    assert pre[y + h, x + h] == pytest.approx(N / J, rel=2e-4)
+
+
+def test_point_source_geometry_cases_cover_both_v22_signs(oracle):
+    signs = {np.sign(_geometry_case(oracle, s)["V"][2, 2]) for s in GEOM_SEEDS}
+    assert signs == {-1.0, 1.0}
+
+
+@pytest.mark.parametrize("seed", GEOM_SEEDS)
+def test_point_source_pre_image_peak_in_pca_frame(oracle, seed):
+    """fo_project + fo_grid + FFT + crop: the pre-image peak is at
+    (V0 . s', V1 . s') / cell + h for a random tilted frame."""
+    c = _geometry_case(oracle, seed)
+    image, cell = 256, 5.8e-5
+    h = image // 2
+    s = np.array([c["l"], c["m"], c["n"] - 1])
+    V = c["V"].astype(np.float64)
+    r, col = _peak(c["pre"])
+    assert abs(r - (V[0] @ s / cell + h)) < 0.5
+    assert abs(col - (V[1] @ s / cell + h)) < 0.5
+
+
+@pytest.mark.parametrize("seed", GEOM_SEEDS)
+def test_point_source_final_peak_on_sky_position(oracle, seed):
+    """fo_coordschange + fo_p2p + fo_splat: the re-projected peak sits at row
+    sign(V22) l / cell + h, column -sign(V22) m / cell + h for every tilted V
+    (FIPkernels.cu:289-299, 378-413).  A swapped V01/V10, a dropped / |V22|
+    or a signed V22 in the divisor moves it by >= 3 px."""
+    c = _geometry_case(oracle, seed)
+    image, cell = 256, 5.8e-5
+    h = image // 2
+    sg = np.sign(c["V"][2, 2])
+    r, col = _peak(c["dirty"])
+    assert abs(r - (sg * c["l"] / cell + h)) < 0.6, (r, sg * c["l"] / cell + h)
+    assert abs(col - (-sg * c["m"] / cell + h)) < 0.6, (col, -sg * c["m"] / cell + h)
+
+
+def _splat_one(oracle, LL, MM, image=64, nbl=4, val=3.0, src=(5, 9)):
+    """fo_splat of one source (column i, row j = src) placed at (LL, MM)."""
+    h = image // 2
+    idx = np.full((image, image, 2), 1e6)          # every other source outside
+    idx[src[0], src[1]] = (LL + h, MM + h)
+    pre = np.zeros((image, image))
+    pre[src[1], src[0]] = val                      # pre[row j, column i]
+    return oracle.splat(idx, pre, nbl), h, val / nbl
+
+
+@pytest.mark.parametrize("LL,MM", [(10.25, -7.625), (-3.625, 4.25), (0.75, 0.125), (-20.5, -0.875)])
+def test_splat_subpixel_bilinear_corners(oracle, LL, MM):
+    """finalinterp (FIPkernels.cu:385-410): a source at fractional (LL, MM)
+    puts (1-a)(1-b), (1-a)b, a(1-b), ab of val/N on (fM,fL), (cM,fL),
+    (fM,cL), (cM,cL) with a = {LL}, b = {MM}; rows follow MM, columns LL.
+    Equivalent invariants: the mass is val/N, the centroid is exactly
+    (row MM + h, column LL + h), and the 2 x 2 footprint is rank one."""
+    out, h, v = _splat_one(oracle, LL, MM)
+    a, b = LL - np.floor(LL), MM - np.floor(MM)
+    fL, fM = int(np.floor(LL)) + h, int(np.floor(MM)) + h
+    blk = out[fM:fM + 2, fL:fL + 2]
+    assert out.sum() == pytest.approx(v, rel=1e-14)
+    assert blk.sum() == pytest.approx(v, rel=1e-14)          # nothing outside the 2 x 2 block
+    rows, cols = np.mgrid[0:out.shape[0], 0:out.shape[1]]
+    assert (out * rows).sum() / v == pytest.approx(MM + h, abs=1e-12)
+    assert (out * cols).sum() / v == pytest.approx(LL + h, abs=1e-12)
+    assert abs(np.linalg.det(blk)) < 1e-14 * v * v
+    expect = v * np.array([[(1 - a) * (1 - b), a * (1 - b)],
+                           [(1 - a) * b, a * b]])
+    assert np.abs(blk - expect).max() < 1e-15
+
+
+def test_splat_first_moments_are_linear(oracle):
+    """Many sources at random sub-pixel positions: the output's first moments
+    equal the value-weighted mean target position (bilinear splatting keeps
+    the centroid of every source)."""
+    image, nbl = 64, 2
+    rng = np.random.default_rng(9)
+    idx = rng.uniform(6, image - 6, (image, image, 2))
+    pre = rng.uniform(0, 1, (image, image))
+    out = oracle.splat(idx, pre, nbl)
+    w = pre.T / nbl                                # source (i, j) carries pre[j, i]
+    rows, cols = np.mgrid[0:image, 0:image]
+    assert out.sum() == pytest.approx(w.sum(), rel=1e-13)
+    assert (out * cols).sum() == pytest.approx((w * idx[..., 0]).sum(), rel=1e-13)
+    assert (out * rows).sum() == pytest.approx((w * idx[..., 1]).sum(), rel=1e-13)
