@@ -84,10 +84,11 @@ int pipe_host(const void* vis_re, const void* vis_im, int vt, const float* bin, 
     FIP_HIP(C->in_b.ensure(bbytes));
     FIP_HIP(C->in_v.ensure(mbytes));
     hipStream_t st = C->stream;
+    if (int rc = C->order_work(st)) return rc;
     // Host -> pinned staging -> device in pieces of up to 4 MiB: the DMA of
     // one piece overlaps the host copy of the next (1 MiB pieces measured
-    // slower: more copy launches).  The staging buffer is free: the previous
-    // call ended with a stream synchronisation.
+    // slower: more copy launches).  The staging buffer is free: every return
+    // after the first copy below synchronises the stream first.
     const struct {
         const void* src;
         char* stage;
@@ -98,25 +99,32 @@ int pipe_host(const void* vis_re, const void* vis_im, int vt, const float* bin, 
                   {bin, hs + 2 * vbytes, C->in_b.p, bbytes},
                   {vin, hs + 2 * vbytes + bbytes, C->in_v.p, mbytes}};
     constexpr size_t kPiece = size_t(4) << 20;
-    for (const auto& pt : parts) {
-        for (size_t o = 0; o < pt.bytes; o += kPiece) {
-            const size_t n = std::min(kPiece, pt.bytes - o);
-            std::memcpy(pt.stage + o, static_cast<const char*>(pt.src) + o, n);
-            FIP_HIP(hipMemcpyAsync(static_cast<char*>(pt.dst) + o, pt.stage + o, n, hipMemcpyHostToDevice, st));
-        }
-    }
-    if (int rc = C->work.ensure(S, nbl, P->t)) return rc;
-    if (int rc = ensure_win012(*C)) return rc;
-    FIP_HIP(C->out.ensure(rbytes));
-    InputView in{C->in_vr.p, C->in_vi.p, C->in_b.as<float>(), C->in_v.as<float>(), nullptr, nbl, (int)nbl, vt};
-    if (int rc = run_imaging(*C, *P, C->work, in, S, (int)nbl, cell, nullptr, (int)nbl, C->tuning(image), st))
-        return rc;
-    if (int rc = run_trigger(*C, *P, C->work, nbl, C->win.as<int>(), 1, (int)tile, C->out.as<float>(), st))
-        return rc;
     char* hr = hs + 2 * vbytes + bbytes + mbytes;
-    FIP_HIP(hipMemcpyAsync(hr, C->out.p, rbytes, hipMemcpyDeviceToHost, st));
-    FIP_HIP(hipMemcpyAsync(hr + rbytes, C->work.meta.p, S * sizeof(SnapMeta), hipMemcpyDeviceToHost, st));
-    FIP_HIP(hipStreamSynchronize(st));
+    auto enqueue = [&]() -> int {
+        for (const auto& pt : parts) {
+            for (size_t o = 0; o < pt.bytes; o += kPiece) {
+                const size_t n = std::min(kPiece, pt.bytes - o);
+                std::memcpy(pt.stage + o, static_cast<const char*>(pt.src) + o, n);
+                FIP_HIP(hipMemcpyAsync(static_cast<char*>(pt.dst) + o, pt.stage + o, n, hipMemcpyHostToDevice, st));
+            }
+        }
+        if (int rc = C->work.ensure(S, nbl, P->t)) return rc;
+        if (int rc = ensure_win012(*C)) return rc;
+        FIP_HIP(C->out.ensure(rbytes));
+        InputView in{C->in_vr.p, C->in_vi.p, C->in_b.as<float>(), C->in_v.as<float>(), nullptr, nbl, (int)nbl, vt};
+        if (int rc = run_imaging(*C, *P, C->work, in, S, (int)nbl, cell, nullptr, (int)nbl, C->tuning(image), st))
+            return rc;
+        if (int rc = run_trigger(*C, *P, C->work, nbl, C->win.as<int>(), 1, (int)tile, C->out.as<float>(), st))
+            return rc;
+        FIP_HIP(hipMemcpyAsync(hr, C->out.p, rbytes, hipMemcpyDeviceToHost, st));
+        FIP_HIP(hipMemcpyAsync(hr + rbytes, C->work.meta.p, S * sizeof(SnapMeta), hipMemcpyDeviceToHost, st));
+        return C->mark_work(st);
+    };
+    const int erc = enqueue();
+    // Also on an error: copies out of the staging buffer may be in flight.
+    const hipError_t se = hipStreamSynchronize(st);
+    if (erc) return erc;
+    if (se != hipSuccess) return hip_error(se, "hipStreamSynchronize");
     if (int rc = C->collect_stages()) return rc;
     std::memcpy(result, hr, rbytes);
     SnapMeta meta[3];
@@ -209,12 +217,14 @@ int fip_pipe_device(const float* vis_re, const float* vis_im, const float* bin, 
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : C->stream;
         if (int rc = C->work.ensure(3, num_baselines, P->t)) return rc;
         if (int rc = ensure_win012(*C)) return rc;
+        if (int rc = C->order_work(st)) return rc;
         InputView in{vis_re, vis_im, bin, vin, nullptr, num_baselines, (int)num_baselines, FIP_VIS_F32};
         if (int rc = run_imaging(*C, *P, C->work, in, 3, (int)num_baselines, cell_size, nullptr,
                                  (int)num_baselines, C->tuning(image_size), st))
             return rc;
         if (int rc = run_trigger(*C, *P, C->work, num_baselines, C->win.as<int>(), 1, (int)unit_size, result, st))
             return rc;
+        if (int rc = C->mark_work(st)) return rc;
         if (C->profiling) {
             FIP_HIP(hipStreamSynchronize(st));
             return C->collect_stages();
@@ -246,6 +256,8 @@ int fip_image(const float* vis_re, const float* vis_im, const float* bin, const 
         FIP_HIP(C->in_b.ensure(bb));
         FIP_HIP(C->in_v.ensure(mb));
         hipStream_t st = C->stream;
+        if (int rc = C->order_work(st)) return rc;
+        FIP_HIP(hipStreamSynchronize(st));
         FIP_HIP(hipMemcpy(C->in_vr.p, vis_re, vb, hipMemcpyHostToDevice));
         FIP_HIP(hipMemcpy(C->in_vi.p, vis_im, vb, hipMemcpyHostToDevice));
         FIP_HIP(hipMemcpy(C->in_b.p, bin, bb, hipMemcpyHostToDevice));
@@ -257,6 +269,7 @@ int fip_image(const float* vis_re, const float* vis_im, const float* bin, const 
                                  C->tuning(image_size), st))
             return rc;
         FIP_HIP(launch_max_finalize(C->work.view(nbl), S, st));
+        if (int rc = C->mark_work(st)) return rc;
         FIP_HIP(hipStreamSynchronize(st));
         if (int rc = C->collect_stages()) return rc;
         const size_t ib = (size_t)S * image_size * image_size * sizeof(float);
@@ -288,6 +301,8 @@ int fip_trigger(const float* d1, const float* d2, const float* d3, const float m
         if (int rc = C->plan(image_size, &P)) return rc;
         if (int rc = C->work.ensure(3, 1, P->t)) return rc;
         if (int rc = ensure_win012(*C)) return rc;
+        if (int rc = C->order_work(C->stream)) return rc;
+        FIP_HIP(hipStreamSynchronize(C->stream));
         const size_t ib = image_size * image_size * sizeof(float);
         float* img = C->work.img.as<float>();
         FIP_HIP(hipMemcpy(img, d1, ib, hipMemcpyHostToDevice));
@@ -299,14 +314,15 @@ int fip_trigger(const float* d1, const float* d2, const float* d3, const float m
         FIP_HIP(hipMemcpy(C->work.meta.p, meta, sizeof(meta), hipMemcpyHostToDevice));
         // The trigger reads the image maxima from the spread slots: slot 0
         // holds maxall[s], the others zero.
-        std::vector<unsigned> slots(3 * kPmaxPer, 0u);
-        for (int s = 0; s < 3; ++s) std::memcpy(&slots[(size_t)s * kPmaxPer + kMaxSlots * kSlotStride], &maxall[s], 4);
+        std::vector<unsigned> slots(pmax_words(3), 0u);
+        for (int s = 0; s < 3; ++s) std::memcpy(&slots[pmax_index(s, 1, 0)], &maxall[s], 4);
         FIP_HIP(hipMemcpy(C->work.pmax.p, slots.data(), slots.size() * sizeof(unsigned), hipMemcpyHostToDevice));
         const size_t un = image_size / unit_size;
         FIP_HIP(C->out.ensure(un * un * sizeof(float)));
         if (int rc = run_trigger(*C, *P, C->work, 1, C->win.as<int>(), 1, (int)unit_size, C->out.as<float>(),
                                  C->stream))
             return rc;
+        if (int rc = C->mark_work(C->stream)) return rc;
         FIP_HIP(hipStreamSynchronize(C->stream));
         if (int rc = C->collect_stages()) return rc;
         FIP_HIP(hipMemcpy(result, C->out.p, un * un * sizeof(float), hipMemcpyDeviceToHost));
@@ -334,6 +350,7 @@ struct BatchKey {
     uint32_t cell_bits;
     int t0, t1, t2, t3;
     int nmax;
+    unsigned long long work_gen;   // the work buffers' addresses are baked into the graph
     bool operator==(const BatchKey& o) const { return std::memcmp(this, &o, sizeof(BatchKey)) == 0; }
 };
 
@@ -414,9 +431,11 @@ int fip_batch_device(const void* vis_re, const void* vis_im, int vis_type, const
             // windows (k, k+1, k+2): the run form reads each image once per 8 windows
             return run_trigger_run(*C, *P, C->work, stride, 0, S - 2, (int)unit_size, d_tlisi, st);
         };
+        if (int rc = C->order_work(st)) return rc;
         if (C->profiling) {
             // Eager launches with per-stage events (no graph).
             if (int rc = enqueue()) return rc;
+            if (int rc = C->mark_work(st)) return rc;
             FIP_HIP(hipStreamSynchronize(st));
             return C->collect_stages();
         }
@@ -440,6 +459,7 @@ int fip_batch_device(const void* vis_re, const void* vis_im, int vis_type, const
         key.t2 = tu.splat_tile;
         key.t3 = tu.persistent;
         key.nmax = nmax;   // sizes the bin/scatter grids baked into the graph
+        key.work_gen = C->work.gen;
         if (!bg.valid || !(bg.key == key)) {
             if (bg.exec) (void)hipGraphExecDestroy(bg.exec);
             if (bg.graph) (void)hipGraphDestroy(bg.graph);
@@ -461,7 +481,7 @@ int fip_batch_device(const void* vis_re, const void* vis_im, int vis_type, const
             bg.valid = true;
         }
         FIP_HIP(hipGraphLaunch(bg.exec, st));
-        return FIP_OK;
+        return C->mark_work(st);
     } catch (...) {
         return set_error(FIP_ERR_HIP, "unexpected exception");
     }
@@ -489,6 +509,7 @@ int fip_batch_host(const void* vis_re, const void* vis_im, int vis_type, const f
         FIP_HIP(C->in_b.ensure(bb));
         FIP_HIP(C->in_v.ensure(mb));
         FIP_HIP(C->out.ensure(ob));
+        if (int rc = C->order_work(C->stream)) return rc;
         FIP_HIP(hipStreamSynchronize(C->stream));
         FIP_HIP(hipMemcpy(C->in_vr.p, vis_re, vb, hipMemcpyHostToDevice));
         FIP_HIP(hipMemcpy(C->in_vi.p, vis_im, vb, hipMemcpyHostToDevice));
@@ -596,6 +617,8 @@ int fip_autotune(size_t image_size, size_t num_baselines, float cell_size, int r
             std::copy(vi.begin() + s * nall, vi.begin() + s * nall + nbl, pi.begin() + s * nbl);
             std::copy(b.begin() + 2 * s * nall, b.begin() + 2 * (s * nall + nbl), pb.begin() + 2 * s * nbl);
         }
+        if (int rc = C->order_work(C->stream)) return rc;
+        FIP_HIP(hipStreamSynchronize(C->stream));
         FIP_HIP(C->in_vr.ensure(S * nbl * 4));
         FIP_HIP(C->in_vi.ensure(S * nbl * 4));
         FIP_HIP(C->in_b.ensure(S * nbl * 8));
@@ -674,6 +697,7 @@ int fip_autotune(size_t image_size, size_t num_baselines, float cell_size, int r
         std::copy(saved_ms, saved_ms + kStCount, C->stage_ms);
         std::copy(saved_n, saved_n + kStCount, C->stage_n);
         C->profiling = was;
+        if (int rc = C->mark_work(C->stream)) return rc;
         C->set_tuning(image_size, best);
         if (choice) {
             choice[0] = best.stage1_records;

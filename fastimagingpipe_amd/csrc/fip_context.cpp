@@ -4,6 +4,7 @@
 #include "fip.h"
 #include "fip_plan.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -77,31 +78,53 @@ void PinnedBuf::release()
 
 // ---------------------------------------------------------------- work
 
-size_t Work::bytes(int S, size_t stride, const PlanTables& p) const
+namespace {
+
+constexpr int kWorkBufs = 12;
+
+// Bytes of each work buffer for S snapshots, in the order of Work::bufs().
+void work_sizes(int S, size_t stride, const PlanTables& p, size_t out[kWorkBufs])
 {
     const size_t s = (size_t)S;
     const size_t nvis = s * stride;
-    return nvis * sizeof(VisRec) * 2 + nvis * sizeof(int2) + s * p.G * 4 + s * (p.G + 1) * 4 +
-           s * sizeof(SnapMeta) + s * sizeof(SnapGeom) + s * (size_t)p.I * (p.M + 1) * 8 +
-           2 * s * (size_t)p.I * p.I * 4 + 256 + s * kPmaxPer * sizeof(unsigned);
+    const size_t sizes[kWorkBufs] = {nvis * sizeof(VisRec),                          // rec
+                                     nvis * sizeof(int2),                            // bkslot
+                                     nvis * sizeof(VisRec),                          // sorted
+                                     s * p.G * sizeof(int),                          // hist
+                                     s * (p.G + 1) * sizeof(int),                    // bstart
+                                     s * sizeof(SnapMeta),                           // meta
+                                     s * sizeof(SnapGeom),                           // geom
+                                     s * (size_t)p.I * (p.M + 1) * sizeof(float2),   // kt
+                                     s * (size_t)p.I * p.I * sizeof(float),          // pre
+                                     s * (size_t)p.I * p.I * sizeof(float),          // img
+                                     256,                                            // work
+                                     pmax_words(S) * sizeof(unsigned)};              // pmax
+    std::copy(sizes, sizes + kWorkBufs, out);
+}
+
+}  // namespace
+
+size_t Work::bytes(int S, size_t stride, const PlanTables& p) const
+{
+    size_t sz[kWorkBufs];
+    work_sizes(S, stride, p, sz);
+    size_t total = 0;
+    for (size_t b : sz) total += b;
+    return total;
 }
 
 int Work::ensure(int S, size_t stride, const PlanTables& p)
 {
-    const size_t s = (size_t)S;
-    const size_t nvis = s * stride;
-    FIP_HIP(rec.ensure(nvis * sizeof(VisRec)));
-    FIP_HIP(sorted.ensure(nvis * sizeof(VisRec)));
-    FIP_HIP(bkslot.ensure(nvis * sizeof(int2)));
-    FIP_HIP(hist.ensure(s * p.G * sizeof(int)));
-    FIP_HIP(bstart.ensure(s * (p.G + 1) * sizeof(int)));
-    FIP_HIP(meta.ensure(s * sizeof(SnapMeta)));
-    FIP_HIP(geom.ensure(s * sizeof(SnapGeom)));
-    FIP_HIP(kt.ensure(s * (size_t)p.I * (p.M + 1) * sizeof(float2)));
-    FIP_HIP(pre.ensure(s * (size_t)p.I * p.I * sizeof(float)));
-    FIP_HIP(img.ensure(s * (size_t)p.I * p.I * sizeof(float)));
-    FIP_HIP(work.ensure(256));
-    FIP_HIP(pmax.ensure(s * kPmaxPer * sizeof(unsigned)));
+    DevBuf* const all[kWorkBufs] = {&rec, &bkslot, &sorted, &hist, &bstart, &meta,
+                                    &geom, &kt, &pre, &img, &work, &pmax};
+    size_t want[kWorkBufs];
+    work_sizes(S, stride, p, want);
+    for (int i = 0; i < kWorkBufs; ++i) {
+        void* const was = all[i]->p;
+        const hipError_t e = all[i]->ensure(want[i]);
+        if (all[i]->p != was) ++gen;
+        if (e != hipSuccess) return hip_error(e, "work buffer allocation");
+    }
     return FIP_OK;
 }
 
@@ -128,6 +151,7 @@ void Work::release()
 {
     for (DevBuf* b : {&rec, &bkslot, &sorted, &hist, &bstart, &meta, &geom, &kt, &pre, &img, &work, &pmax})
         b->release();
+    ++gen;
 }
 
 // ------------------------------------------------------------- context
@@ -147,6 +171,7 @@ int Context::get(Context** out)
         auto* c = new Context();
         FIP_HIP(hipGetDevice(&c->device));
         FIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        FIP_HIP(hipEventCreateWithFlags(&c->work_done_, hipEventDisableTiming));
         for (auto& pair : c->ev_) {
             FIP_HIP(hipEventCreate(&pair[0]));
             FIP_HIP(hipEventCreate(&pair[1]));
@@ -211,6 +236,19 @@ Tuning Context::tuning(size_t image) const
     t.splat_tile = 64;
     t.persistent = 0;
     return t;
+}
+
+int Context::order_work(hipStream_t st)
+{
+    if (work_stream_ && work_stream_ != st) FIP_HIP(hipStreamWaitEvent(st, work_done_, 0));
+    return FIP_OK;
+}
+
+int Context::mark_work(hipStream_t st)
+{
+    FIP_HIP(hipEventRecord(work_done_, st));
+    work_stream_ = st;
+    return FIP_OK;
 }
 
 int Context::begin_stage(int st, hipStream_t s)

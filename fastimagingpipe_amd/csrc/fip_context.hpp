@@ -48,6 +48,9 @@ struct Plan {
 // Work buffers sized for S snapshots of one image size and stride.
 struct Work {
     DevBuf rec, bkslot, sorted, hist, bstart, meta, geom, kt, pre, img, work, pmax;
+    // Bumped whenever any buffer is reallocated or released: a captured graph
+    // that baked in the old addresses must not be replayed after that.
+    unsigned long long gen = 0;
     WorkView view(size_t stride) const;
     int ensure(int S, size_t stride, const PlanTables& p);
     size_t bytes(int S, size_t stride, const PlanTables& p) const;
@@ -78,11 +81,20 @@ public:
     DevBuf in_vr, in_vi, in_b, in_v, out, win, nvis, norm;
     PinnedBuf staging;
 
+    // Every entry point that enqueues work on `work` from some stream calls
+    // order_work(st) before and mark_work(st) after, so uses of the shared
+    // buffers from different streams (the context's own and callers') run
+    // one after another in call order.
+    int order_work(hipStream_t st);
+    int mark_work(hipStream_t st);
+
 private:
     std::map<size_t, std::unique_ptr<Plan>> plans_;
     std::map<size_t, Tuning> tunings_;
     hipEvent_t ev_[kStCount][2] = {};
     bool ev_used_[kStCount] = {};
+    hipEvent_t work_done_ = nullptr;     // recorded after the last enqueue on `work`
+    hipStream_t work_stream_ = nullptr;  // the stream of that enqueue
 };
 
 // Enqueue reset..splat for S snapshots described by `in` into `w`.

@@ -27,6 +27,10 @@ namespace {
 
 constexpr size_t kBlock = 2880;
 constexpr size_t kCard = 80;
+// Largest accepted NAXISn, PCOUNT and GCOUNT (2^40): larger values cannot
+// describe a file this reader could hold, and the bound keeps the size
+// arithmetic far from overflow.
+constexpr long long kMaxCount = 1ll << 40;
 
 struct Hdu {
     int bitpix = 0;
@@ -141,7 +145,7 @@ size_t parse_header(const std::vector<unsigned char>& f, size_t off, bool primar
                 have_naxis = true;
             } else if (key.size() > 5 && key.compare(0, 5, "NAXIS") == 0) {
                 long long k = 0;
-                if (!parse_long(key.substr(5), &k) || !parse_long(val, &iv) || iv < 0) {
+                if (!parse_long(key.substr(5), &k) || !parse_long(val, &iv) || iv < 0 || iv > kMaxCount) {
                     err = "bad " + key;
                     return 0;
                 }
@@ -151,9 +155,15 @@ size_t parse_header(const std::vector<unsigned char>& f, size_t off, bool primar
                     return 0;
                 }
             } else if (key == "PCOUNT") {
-                parse_long(val, &h.pcount);
+                if (!parse_long(val, &h.pcount) || h.pcount < 0 || h.pcount > kMaxCount) {
+                    err = "bad PCOUNT";
+                    return 0;
+                }
             } else if (key == "GCOUNT") {
-                parse_long(val, &h.gcount);
+                if (!parse_long(val, &h.gcount) || h.gcount < 1 || h.gcount > kMaxCount) {
+                    err = "bad GCOUNT";
+                    return 0;
+                }
             } else if (key == "BSCALE") {
                 parse_double(val, &h.bscale);
             } else if (key == "BZERO") {
@@ -173,16 +183,25 @@ size_t parse_header(const std::vector<unsigned char>& f, size_t off, bool primar
         return 0;
     }
     for (int k = 0; k < 3; ++k) h.naxes[k] = k < h.naxis ? (long)naxis_n[k] : 1;
-    unsigned long long n = h.naxis == 0 ? 0ull : 1ull;
-    for (int k = 0; k < h.naxis && k < 3; ++k) n *= (unsigned long long)naxis_n[k];
+    // Data size |BITPIX|/8 * GCOUNT * (PCOUNT + prod NAXISn), every step
+    // checked: a crafted header must not wrap it into a small number.
+    unsigned long long n = h.naxis == 0 ? 0ull : 1ull, bytes = 0;
+    bool ovf = false;
+    for (int k = 0; k < h.naxis && k < 3; ++k) ovf |= __builtin_mul_overflow(n, (unsigned long long)naxis_n[k], &n);
+    ovf |= __builtin_add_overflow(n, (unsigned long long)h.pcount, &bytes);
+    ovf |= __builtin_mul_overflow(bytes, (unsigned long long)h.gcount, &bytes);
+    ovf |= __builtin_mul_overflow(bytes, (unsigned long long)(std::abs(b) / 8), &bytes);
+    if (ovf || bytes > (unsigned long long)(SIZE_MAX / 2)) {
+        err = "FITS data size overflows";
+        return 0;
+    }
     h.data_offset = pos;
-    h.data_bytes = (size_t)((unsigned long long)(std::abs(b) / 8) * (unsigned long long)std::max(1ll, h.gcount) *
-                            ((unsigned long long)h.pcount + n));
-    const size_t padded = (h.data_bytes + kBlock - 1) / kBlock * kBlock;
-    if (h.data_offset + h.data_bytes > f.size()) {
+    h.data_bytes = (size_t)bytes;
+    if (h.data_offset > f.size() || h.data_bytes > f.size() - h.data_offset) {
         err = "truncated FITS data";
         return 0;
     }
+    const size_t padded = (h.data_bytes + kBlock - 1) / kBlock * kBlock;
     return h.data_offset + padded;
 }
 
@@ -234,7 +253,11 @@ int read_hdu(const char* path, const char* extname, float** data, long dims[3])
                                         : (h.xtension == "IMAGE" && h.extname == std::string(extname));
         if (match) {
             if (h.naxis > 3) return fip::set_error(FIP_ERR_IO, std::string(path) + ": NAXIS > 3");
+            // parse_header checked that the product of the axes and its byte
+            // size do not overflow and lie inside the file.
             const size_t n = h.naxis == 0 ? 0 : (size_t)h.naxes[0] * h.naxes[1] * h.naxes[2];
+            if (n > (f.size() - h.data_offset) / (size_t)(std::abs(h.bitpix) / 8))
+                return fip::set_error(FIP_ERR_IO, std::string(path) + ": truncated FITS data");
             float* out = static_cast<float*>(std::malloc(std::max<size_t>(n, 1) * sizeof(float)));
             if (!out) return fip::set_error(FIP_ERR_ALLOC, "host allocation failed");
             const unsigned char* p = &f[h.data_offset];

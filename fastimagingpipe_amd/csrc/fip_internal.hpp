@@ -39,12 +39,11 @@ struct VisRec {
 // address, not per cache line (slots 128 bytes apart measured the same for
 // the writers), so the slots are packed and a wave reads them in two lines.
 constexpr int kMaxSlots = 64;
-constexpr int kSlotStride = 1;                                   // unsigned per slot
-constexpr size_t kPmaxPer = (size_t)2 * kMaxSlots * kSlotStride;   // unsigned per snapshot
-__host__ __device__ inline unsigned* max_slot(unsigned* pm, int s, int set, int slot)
-{
-    return pm + (size_t)s * kPmaxPer + ((size_t)set * kMaxSlots + slot) * kSlotStride;
-}
+// The slot layout is defined once, in the kernels' translation unit: host
+// code sizes and fills slot arrays only through these two functions, so an
+// allocation can never disagree with the kernels' indexing.
+size_t pmax_words(int S);                       // unsigned words for S snapshots
+size_t pmax_index(int s, int set, int slot);    // word of one slot
 
 // Per-snapshot scalars produced and consumed on the device.
 struct SnapMeta {
@@ -108,7 +107,7 @@ struct WorkView {
     float* pre;        // [S][I][I] corrected image before re-projection
     float* img;        // [S][I][I] re-projected images
     unsigned* work;    // [1] persistent-kernel work counter
-    unsigned* pmax;    // [S][kPmaxPer] spread slots of the image maxima
+    unsigned* pmax;    // [pmax_words(S)] spread slots of the image maxima
     size_t stride;
 };
 

@@ -38,6 +38,19 @@ namespace fip {
 
 // ---------------------------------------------------------------- helpers
 
+// Spread-slot layout of the image maxima: [S][set 0..1][kMaxSlots] unsigned.
+constexpr size_t kPmaxPer = (size_t)2 * kMaxSlots;   // unsigned per snapshot
+__host__ __device__ inline size_t max_slot_index(int s, int set, int slot)
+{
+    return (size_t)s * kPmaxPer + (size_t)set * kMaxSlots + (size_t)slot;
+}
+__device__ __forceinline__ unsigned* max_slot(unsigned* pm, int s, int set, int slot)
+{
+    return pm + max_slot_index(s, set, slot);
+}
+size_t pmax_words(int S) { return (size_t)S * kPmaxPer; }
+size_t pmax_index(int s, int set, int slot) { return max_slot_index(s, set, slot); }
+
 // exp(beta*(sqrt(1-x^2)-1)) in the cancellation-free form
 // exp(-beta*x^2/(1+sqrt(1-x^2))) (FIPkernels.cu:67-71).
 __device__ __forceinline__ float es_f(float x)
@@ -443,13 +456,16 @@ __device__ __forceinline__ void grid_taps(const ColRec& c, int M, int G, int rbs
 }
 
 // Adds the record's magnitude bound (units of amax / 4096, rounded up) to the
-// mass of every scale block its taps touch (at most two).
-__device__ __forceinline__ void tap_mass(const ColRec& c, int M, int G, int rbs, float qs, unsigned* mass)
+// mass of every scale block its taps touch (at most two).  The mass is 64-bit:
+// a column holds up to two records per visibility (its direct and mirrored
+// buckets), so a 32-bit sum of up to 4096 units per record would wrap after
+// about 2^19 visibilities at one uv point.
+__device__ __forceinline__ void tap_mass(const ColRec& c, int M, int G, int rbs, float qs, unsigned long long* mass)
 {
     const int lo1 = max(c.b1 - 4 + (c.f1 > 0.0f ? 1 : 0), -M);
     const int hi1 = min(c.b1 + 4, M - 1);
     if (lo1 > hi1) return;
-    const unsigned q = (unsigned)ceilf(fmaxf(fabsf(c.vr), fabsf(c.vi)) * qs);
+    const unsigned long long q = (unsigned long long)ceilf(fmaxf(fabsf(c.vr), fabsf(c.vi)) * qs);
     const int ra = tap_row(c, lo1, M), rb = tap_row(c, hi1, M);
     const int ba = (ra < 0 ? ra + G : ra) / rbs, bb = (rb < 0 ? rb + G : rb) / rbs;
     atomicAdd(&mass[ba], q);
@@ -487,7 +503,7 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
                                               const SnapMeta& m,
                                               const float2* __restrict__ tw,
                                               float2* __restrict__ kt, size_t stride, int I, float2* fb,
-                                              unsigned* mass, float* sc, float* isc)
+                                              unsigned long long* mass, float* sc, float* isc)
 {
     constexpr int NT = stage1_threads<G>();
     constexpr int M = G / 2;
@@ -537,7 +553,7 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
     if (resident) load(0);
     int* col = reinterpret_cast<int*>(fb);
     for (int i = t; i < G; i += NT) fb[lds_pad(i)] = make_float2(0.0f, 0.0f);   // integer zero too
-    for (int i = t; i < kScaleBlocks; i += NT) mass[i] = 0u;
+    for (int i = t; i < kScaleBlocks; i += NT) mass[i] = 0ull;
     __syncthreads();
     for (int base = 0; base < ntot; base += KR * NT) {
         if (!resident) load(base);
@@ -594,7 +610,7 @@ __global__ __launch_bounds__(stage1_threads<G>(), (stage1_threads<G>() >= 1024 ?
                                                             float2* __restrict__ kt, size_t stride, int I)
 {
     __shared__ float2 fb[lds_len(G)];
-    __shared__ unsigned mass[kScaleBlocks];
+    __shared__ unsigned long long mass[kScaleBlocks];
     __shared__ float sc[kScaleBlocks], isc[kScaleBlocks];
     // Column-major over snapshots: the heavily populated low columns of
     // every snapshot are dispatched first, so none of them ends the launch.
@@ -618,7 +634,7 @@ __global__ __launch_bounds__(stage1_threads<G>()) void k_stage1_persistent(
     int I, int S, unsigned* work)
 {
     __shared__ float2 fb[lds_len(G)];
-    __shared__ unsigned mass[kScaleBlocks];
+    __shared__ unsigned long long mass[kScaleBlocks];
     __shared__ float sc[kScaleBlocks], isc[kScaleBlocks];
     __shared__ unsigned item;
     constexpr unsigned per = G / 2 + 1;

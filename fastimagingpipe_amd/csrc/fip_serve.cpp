@@ -146,7 +146,7 @@ size_t stream_bytes(const fip_stream_config& c, int ring, int q)
     const size_t G = grid_size_for(I), M = G / 2;
     const size_t un = I / c.unit_size;
     const size_t work = N * sizeof(VisRec) * 2 + N * sizeof(int2) + G * 4 + (G + 1) * 4 + I * (M + 1) * 8 + I * I * 4;
-    const size_t ringb = (size_t)ring * (I * I * 4 + sizeof(SnapMeta) + sizeof(SnapGeom) + kPmaxPer * sizeof(unsigned));
+    const size_t ringb = (size_t)ring * (I * I * 4 + sizeof(SnapMeta) + sizeof(SnapGeom)) + pmax_words(ring) * sizeof(unsigned);
     const size_t in = 2 * N * esize(c.vis_type) + N * 8 + 36;
     return work + ringb + un * un * 4 + 256 + (size_t)q * in;
 }
@@ -216,7 +216,7 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
                             std::make_pair(&ssp->img, (size_t)ring * I * I * 4),
                             std::make_pair(&ssp->meta, (size_t)ring * sizeof(SnapMeta)),
                             std::make_pair(&ssp->geom, (size_t)ring * sizeof(SnapGeom)),
-                            std::make_pair(&ssp->pmax, (size_t)ring * kPmaxPer * sizeof(unsigned)),
+                            std::make_pair(&ssp->pmax, pmax_words(ring) * sizeof(unsigned)),
                             std::make_pair(&ssp->win, (size_t)3 * sizeof(int) * ring),
                             std::make_pair(&ssp->out, un * un * 4 * (size_t)q)}) {
                 e = pr.first->ensure(pr.second);
@@ -317,7 +317,7 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     w.pre = ss.pre.as<float>();
     w.img = ss.img.as<float>() + (size_t)slot * I * I;
     w.work = ss.work.as<unsigned>();
-    w.pmax = ss.pmax.as<unsigned>() + (size_t)slot * kPmaxPer;
+    w.pmax = ss.pmax.as<unsigned>() + pmax_index(slot, 0, 0);
     w.stride = N;
     InputView in{sl->vr.p, sl->vi.p, sl->b.as<float>(), sl->v.as<float>(), nullptr, N, (int)N, ss.cfg.vis_type};
     const PlanTables& p = ss.plan->t;

@@ -117,6 +117,43 @@ def test_truncated_and_missing_files(F, tmp_path):
         F.fits_read(str(p))
 
 
+def _raw_header(cards):
+    return _pad("".join([_card(k, v) for k, v in cards] + [_card("END")]).encode("ascii"))
+
+
+@pytest.mark.parametrize("cards", [
+    # negative PCOUNT: |BITPIX|/8 * (PCOUNT + n) would wrap to 0 bytes
+    [("BITPIX", -32), ("NAXIS", 1), ("NAXIS1", 1000), ("PCOUNT", -1000)],
+    # 2^31 x 2^31 x 4 bytes: the unchecked product wraps
+    [("BITPIX", -32), ("NAXIS", 2), ("NAXIS1", 2 ** 31), ("NAXIS2", 2 ** 31)],
+    # 2^62 elements: past the element limit
+    [("BITPIX", 8), ("NAXIS", 1), ("NAXIS1", 2 ** 62)],
+    [("BITPIX", -32), ("NAXIS", 3), ("NAXIS1", 2 ** 21), ("NAXIS2", 2 ** 21), ("NAXIS3", 2 ** 21)],
+    [("BITPIX", 16), ("NAXIS", 1), ("NAXIS1", 10), ("GCOUNT", 0)],
+    [("BITPIX", 16), ("NAXIS", 1), ("NAXIS1", 10), ("GCOUNT", -5)],
+    [("BITPIX", 16), ("NAXIS", 1), ("NAXIS1", 10), ("PCOUNT", "'x'")],
+], ids=["neg_pcount", "wrap_2d", "huge_1d", "wrap_3d", "gcount0", "neg_gcount", "bad_pcount"])
+def test_crafted_headers_are_rejected(F, tmp_path, cards):
+    """Size fields that would overflow or underflow the data size are errors,
+    not short reads or heap overruns."""
+    p = tmp_path / "c.fits"
+    p.write_bytes(_raw_header([("SIMPLE", True)] + cards) + b"\0" * 2880)
+    with pytest.raises(F.FipError) as ei:
+        F.fits_read(str(p))
+    assert ei.value.status == -8
+
+
+def test_crafted_extension_header_is_rejected(F, tmp_path):
+    good = _hdu(np.ones((2, 2), np.float32), -32)
+    bad = _raw_header([("XTENSION", "IMAGE"), ("BITPIX", -32), ("NAXIS", 2), ("NAXIS1", 2 ** 33),
+                       ("NAXIS2", 2 ** 33), ("PCOUNT", 0), ("GCOUNT", 1), ("EXTNAME", "BIG")])
+    p = tmp_path / "e.fits"
+    p.write_bytes(good + bad + b"\0" * 2880)
+    with pytest.raises(F.FipError) as ei:
+        F.fits_read(str(p), "BIG")
+    assert ei.value.status == -8
+
+
 def _run(*args):
     return subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=120)
 

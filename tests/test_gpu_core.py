@@ -64,3 +64,30 @@ def test_trigger_matches_oracle(fip, oracle):
     got = fip.trigger(d[0], d[1], d[2], mx, T)
     ref = oracle.trigger(d[0], d[1], d[2], mx.astype(np.float64), T)
     assert np.abs(got - ref).max() < 1e-5
+
+
+def test_crowded_uv_column_scale_does_not_wrap(fip, oracle):
+    """600000 identical visibilities on the v = 0 axis: Hermitian column 0
+    holds each one twice (direct and mirrored buckets), 1.2M records whose
+    magnitude bounds exceed 2^32 units in one scale block.  The image must
+    still be the known answer N |Re(v e^{2 pi i p1 y / g})| (FIPkernels.cu:197-287)."""
+    image, cell, nbl = 256, 5.8e-5, 600000
+    g = fip.grid_size(image)
+    h = image // 2
+    p1 = 21.0
+    B = np.zeros((3, nbl, 2), np.float32)
+    B[:, :, 0] = np.float32(p1 / (np.float64(np.float32(cell)) * g))
+    vr = np.full((3, nbl), 0.75, np.float32)
+    vi = np.full((3, nbl), -0.25, np.float32)
+    V = np.tile(np.eye(3, dtype=np.float32), (3, 1, 1))
+    pre, _, _ = fip.image(vr, vi, B, V, nbl, image, 3, cell)
+    P1 = np.float64(B[0, 0, 0]) * np.float64(np.float32(cell)) * g
+    y = np.arange(-h, h)
+    row = nbl * np.abs(((0.75 - 0.25j) * np.exp(2j * np.pi * P1 * y / g)).real)
+    kat = np.repeat(row[:, None], image, axis=1)
+    opre, _, _ = oracle.image(vr[0], vi[0], B[0], V[0], image, cell, want_pre=True)
+    e_kat = np.abs(pre[0] - kat).max() / kat.max()
+    e_orc = rel_rms(pre[0], opre)
+    print(f"crowded column: max rel err vs known answer {e_kat:.2e}, rel-rms vs oracle {e_orc:.2e}")
+    assert e_kat < 1e-4
+    assert e_orc < 2e-5
