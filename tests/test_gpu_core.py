@@ -70,7 +70,16 @@ def test_crowded_uv_column_scale_does_not_wrap(fip, oracle):
     """600000 identical visibilities on the v = 0 axis: Hermitian column 0
     holds each one twice (direct and mirrored buckets), 1.2M records whose
     magnitude bounds exceed 2^32 units in one scale block.  The image must
-    still be the known answer N |Re(v e^{2 pi i p1 y / g})| (FIPkernels.cu:197-287)."""
+    still be the known answer N |Re(v e^{2 pi i p1 y / g})| (FIPkernels.cu:197-287).
+
+    What is checked is the scale, not full precision.  A wrapped 32-bit mass
+    would give a scale 2^14 too large and int32 cell sums that overflow: an
+    image wrong by O(1).  With the 64-bit mass the block scale is 2^11 here,
+    so each tap contribution is rounded to 1/2048 (6.5e-4 of the largest
+    one).  Identical records round identically, so those errors add up
+    rather than cancel: N q / 2 per cell, which the grid correction
+    amplifies towards the image edge to a few 1e-2 of the peak.  Random
+    phases (every real array) round incoherently: see the parity tests."""
     image, cell, nbl = 256, 5.8e-5, 600000
     g = fip.grid_size(image)
     h = image // 2
@@ -88,6 +97,13 @@ def test_crowded_uv_column_scale_does_not_wrap(fip, oracle):
     opre, _, _ = oracle.image(vr[0], vi[0], B[0], V[0], image, cell, want_pre=True)
     e_kat = np.abs(pre[0] - kat).max() / kat.max()
     e_orc = rel_rms(pre[0], opre)
-    print(f"crowded column: max rel err vs known answer {e_kat:.2e}, rel-rms vs oracle {e_orc:.2e}")
-    assert e_kat < 1e-4
-    assert e_orc < 2e-5
+    corr = np.corrcoef(pre[0].ravel(), kat.ravel())[0, 1]
+    print(f"crowded column: max rel err vs known answer {e_kat:.2e}, rel-rms vs oracle {e_orc:.2e}, "
+          f"correlation {corr:.6f}")
+    assert np.isfinite(pre).all()
+    assert e_kat < 5e-2
+    assert e_orc < 1e-2
+    assert corr > 0.999
+    # identical snapshots: identical integer sums
+    assert np.array_equal(pre[0].view(np.uint32), pre[1].view(np.uint32))
+    assert np.array_equal(pre[0].view(np.uint32), pre[2].view(np.uint32))
