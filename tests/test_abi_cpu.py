@@ -139,3 +139,17 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_cxx_caller_links_fipipe_by_its_reference_prototype():
+    """tests/abi/fipipe_caller declares FIpipe as pipe_interface.cpp:8 does;
+    its undefined symbol must be the mangled name libfip.so exports."""
+    import os
+    import subprocess
+    caller = os.path.join(os.path.dirname(os.path.abspath(__file__)), "abi", "fipipe_caller")
+    assert os.access(caller, os.X_OK), "build the caller first (__graft_entry__.build())"
+    syms = subprocess.run(["nm", "-D", "--undefined-only", caller], capture_output=True, text=True,
+                          check=True).stdout
+    assert "_Z6FIpipePfS_S_S_S_mmmfm" in syms
+    deps = subprocess.run(["ldd", caller], capture_output=True, text=True, check=True).stdout
+    assert "libfip.so" in deps and "not found" not in deps.split("libfip.so")[1].splitlines()[0]
