@@ -14,14 +14,16 @@ import numpy as np
 
 __all__ = [
     "FipError", "lib", "library_path", "grid_size", "fipipe", "fipipe_lowp", "image",
-    "trigger", "pipe_device", "batch_device", "batch_host", "project", "collapse", "ingest_table",
+    "trigger", "pipe_device", "pipe_raw_device", "batch_device", "batch_host", "project", "collapse", "ingest_table",
     "synth",
     "set_profiling", "stage_times", "reset_stage_times", "autotune", "set_tuning",
     "debug_fft", "fits_read", "fits_write", "fits_write3", "Server", "STATUS",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-library_path = os.path.join(_HERE, "libfip.so")
+# FIP_LIBRARY names another build of the same sources (the host-sanitizer
+# build of tests/sanitize); by default the in-tree library.
+library_path = os.environ.get("FIP_LIBRARY") or os.path.join(_HERE, "libfip.so")
 
 STATUS = {
     0: "FIP_OK", -1: "FIP_ERR_ARG", -2: "FIP_ERR_NSNAP", -3: "FIP_ERR_SIZE",
@@ -94,6 +96,26 @@ _sig("fip_fits_write", [ctypes.c_char_p, _vp, ctypes.c_long, ctypes.c_long])
 _sig("fip_fits_write3", [ctypes.c_char_p, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_long])
 _sig("fip_batch_host", [_vp, _vp, _i, _vp, _vp, _vp, _sz, _sz, _sz, _sz, _f, _sz, _vp])
 _sig("fip_free", [_vp], None)
+
+
+class _RawView(ctypes.Structure):
+    _fields_ = [("uvw", _vp), ("freqs", _vp), ("xx_re", _vp), ("xx_im", _vp), ("yy_re", _vp),
+                ("yy_im", _vp), ("flag0", _vp), ("flag3", _vp), ("w0", _vp), ("w3", _vp),
+                ("nrows", _sz), ("nchan", _sz), ("nsnap", _sz)]
+
+
+_sig("fip_pipe_raw_device", [ctypes.POINTER(_RawView), _sz, _f, _sz, _vp, _vp, _vp, _vp])
+
+
+def pipe_raw_device(ptrs: dict, nrows: int, nchan: int, nsnap: int, image_size: int, cell_size: float,
+                    unit_size: int, out_ptr: int, bin_out_ptr: int = 0, vin_out_ptr: int = 0,
+                    stream: int = 0) -> None:
+    """fip_pipe_raw_device: ``ptrs`` maps uvw, freqs, xx_re, xx_im, yy_re, yy_im, flag0,
+    flag3, w0, w3 to device addresses."""
+    rv = _RawView(**{k: ptrs[k] for k in ("uvw", "freqs", "xx_re", "xx_im", "yy_re", "yy_im", "flag0",
+                                           "flag3", "w0", "w3")}, nrows=nrows, nchan=nchan, nsnap=nsnap)
+    _check(lib.fip_pipe_raw_device(ctypes.byref(rv), image_size, cell_size, unit_size, out_ptr,
+                                   bin_out_ptr or None, vin_out_ptr or None, stream or None))
 
 
 class _StreamConfig(ctypes.Structure):

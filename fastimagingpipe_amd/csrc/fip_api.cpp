@@ -24,25 +24,12 @@ using namespace fip;
 
 namespace {
 
-const int kWin012[3] = {0, 1, 2};
-
 size_t vis_elem_size(int vt) { return vt == FIP_VIS_F32 ? 4 : 2; }
 
 int check_vis_type(int vt)
 {
     if (vt != FIP_VIS_F32 && vt != FIP_VIS_BF16 && vt != FIP_VIS_F16)
         return set_error(FIP_ERR_ARG, "vis_type must be FIP_VIS_F32, FIP_VIS_BF16 or FIP_VIS_F16");
-    return FIP_OK;
-}
-
-// Upload windows {0,1,2} once per context.
-int ensure_win012(Context& C)
-{
-    static bool done = false;
-    if (done && C.win.p) return FIP_OK;
-    FIP_HIP(C.win.ensure(3 * sizeof(int)));
-    FIP_HIP(hipMemcpy(C.win.p, kWin012, sizeof(kWin012), hipMemcpyHostToDevice));
-    done = true;
     return FIP_OK;
 }
 
@@ -74,46 +61,55 @@ int pipe_host(const void* vis_re, const void* vis_im, int vt, const float* bin, 
     if (int rc = C->plan(image, &P)) return rc;
     const int S = 3;   // the reference images snapshots 0..2 only (FIPkernels.cu:647)
     const size_t esz = vis_elem_size(vt);
-    const size_t vbytes = S * nbl * esz, bbytes = S * nbl * 2 * sizeof(float), mbytes = S * 9 * sizeof(float);
+    // One region per snapshot, in pinned staging and on the device:
+    // vis_re | vis_im | bin | vin, each part 256-byte aligned.
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t o_vi = al(nbl * esz), o_b = o_vi + al(nbl * esz), o_v = o_b + al(nbl * 2 * sizeof(float));
+    const size_t region = o_v + al(9 * sizeof(float));
     const size_t un = image / tile;
     const size_t rbytes = un * un * sizeof(float);
-    FIP_HIP(C->staging.ensure(2 * vbytes + bbytes + mbytes + rbytes + S * sizeof(SnapMeta)));
+    FIP_HIP(C->staging.ensure(S * region + rbytes + S * sizeof(SnapMeta)));
     char* hs = static_cast<char*>(C->staging.p);
-    FIP_HIP(C->in_vr.ensure(vbytes));
-    FIP_HIP(C->in_vi.ensure(vbytes));
-    FIP_HIP(C->in_b.ensure(bbytes));
-    FIP_HIP(C->in_v.ensure(mbytes));
-    hipStream_t st = C->stream;
+    FIP_HIP(C->in_snap.ensure(S * region));
+    char* ds = C->in_snap.as<char>();
+    if (int rc = C->work.ensure(S, nbl, P->t)) return rc;
+    if (int rc = C->ensure_win012()) return rc;
+    FIP_HIP(C->out.ensure(rbytes));
+    hipStream_t st = C->stream, cs = C->copy_stream;
     if (int rc = C->order_work(st)) return rc;
-    // Host -> pinned staging -> device in pieces of up to 4 MiB: the DMA of
-    // one piece overlaps the host copy of the next (1 MiB pieces measured
-    // slower: more copy launches).  The staging buffer is free: every return
-    // after the first copy below synchronises the stream first.
-    const struct {
-        const void* src;
-        char* stage;
-        void* dst;
-        size_t bytes;
-    } parts[4] = {{vis_re, hs, C->in_vr.p, vbytes},
-                  {vis_im, hs + vbytes, C->in_vi.p, vbytes},
-                  {bin, hs + 2 * vbytes, C->in_b.p, bbytes},
-                  {vin, hs + 2 * vbytes + bbytes, C->in_v.p, mbytes}};
-    constexpr size_t kPiece = size_t(4) << 20;
-    char* hr = hs + 2 * vbytes + bbytes + mbytes;
+    // Snapshot s+1 is copied (host -> pinned staging -> device, on the copy
+    // stream) while snapshot s is imaged on the compute stream, as the
+    // reference's stream3 does (FIPkernels.cu:648-663, 716-720); the trigger
+    // follows the third image.  Within a snapshot the DMA of one part
+    // overlaps the host copy of the next.  The staging buffer is free: every
+    // return after the first copy below synchronises both streams first.
+    char* hr = hs + S * region;
+    const Tuning tu = C->tuning(image);
+    const WorkView wall = C->work.view(nbl);
     auto enqueue = [&]() -> int {
-        for (const auto& pt : parts) {
-            for (size_t o = 0; o < pt.bytes; o += kPiece) {
-                const size_t n = std::min(kPiece, pt.bytes - o);
-                std::memcpy(pt.stage + o, static_cast<const char*>(pt.src) + o, n);
-                FIP_HIP(hipMemcpyAsync(static_cast<char*>(pt.dst) + o, pt.stage + o, n, hipMemcpyHostToDevice, st));
+        for (int s = 0; s < S; ++s) {
+            const size_t sv = (size_t)s * nbl;
+            const struct {
+                const void* src;
+                size_t off, bytes;
+            } parts[4] = {{static_cast<const char*>(vis_re) + sv * esz, 0, nbl * esz},
+                          {static_cast<const char*>(vis_im) + sv * esz, o_vi, nbl * esz},
+                          {bin + 2 * sv, o_b, nbl * 2 * sizeof(float)},
+                          {vin + 9 * (size_t)s, o_v, 9 * sizeof(float)}};
+            char* h = hs + (size_t)s * region;
+            char* d = ds + (size_t)s * region;
+            for (const auto& pt : parts) {
+                std::memcpy(h + pt.off, pt.src, pt.bytes);
+                FIP_HIP(hipMemcpyAsync(d + pt.off, h + pt.off, pt.bytes, hipMemcpyHostToDevice, cs));
             }
+            FIP_HIP(hipEventRecord(C->in_ready[s], cs));
+            FIP_HIP(hipStreamWaitEvent(st, C->in_ready[s], 0));
+            InputView in{d, d + o_vi, reinterpret_cast<const float*>(d + o_b), reinterpret_cast<const float*>(d + o_v),
+                         nullptr, nbl, (int)nbl, vt};
+            if (int rc = imaging_sequence(C, P->t, snapshot_view(wall, s, P->t), in, 1, (int)nbl, cell, nullptr,
+                                          (int)nbl, tu, st))
+                return rc;
         }
-        if (int rc = C->work.ensure(S, nbl, P->t)) return rc;
-        if (int rc = ensure_win012(*C)) return rc;
-        FIP_HIP(C->out.ensure(rbytes));
-        InputView in{C->in_vr.p, C->in_vi.p, C->in_b.as<float>(), C->in_v.as<float>(), nullptr, nbl, (int)nbl, vt};
-        if (int rc = run_imaging(*C, *P, C->work, in, S, (int)nbl, cell, nullptr, (int)nbl, C->tuning(image), st))
-            return rc;
         if (int rc = run_trigger(*C, *P, C->work, nbl, C->win.as<int>(), 1, (int)tile, C->out.as<float>(), st))
             return rc;
         FIP_HIP(hipMemcpyAsync(hr, C->out.p, rbytes, hipMemcpyDeviceToHost, st));
@@ -122,8 +118,10 @@ int pipe_host(const void* vis_re, const void* vis_im, int vt, const float* bin, 
     };
     const int erc = enqueue();
     // Also on an error: copies out of the staging buffer may be in flight.
+    const hipError_t sc = hipStreamSynchronize(cs);
     const hipError_t se = hipStreamSynchronize(st);
     if (erc) return erc;
+    if (sc != hipSuccess) return hip_error(sc, "hipStreamSynchronize (copy stream)");
     if (se != hipSuccess) return hip_error(se, "hipStreamSynchronize");
     if (int rc = C->collect_stages()) return rc;
     std::memcpy(result, hr, rbytes);
@@ -216,7 +214,7 @@ int fip_pipe_device(const float* vis_re, const float* vis_im, const float* bin, 
         if (int rc = C->plan(image_size, &P)) return rc;
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : C->stream;
         if (int rc = C->work.ensure(3, num_baselines, P->t)) return rc;
-        if (int rc = ensure_win012(*C)) return rc;
+        if (int rc = C->ensure_win012()) return rc;
         if (int rc = C->order_work(st)) return rc;
         InputView in{vis_re, vis_im, bin, vin, nullptr, num_baselines, (int)num_baselines, FIP_VIS_F32};
         if (int rc = run_imaging(*C, *P, C->work, in, 3, (int)num_baselines, cell_size, nullptr,
@@ -300,7 +298,7 @@ int fip_trigger(const float* d1, const float* d2, const float* d3, const float m
         Plan* P = nullptr;
         if (int rc = C->plan(image_size, &P)) return rc;
         if (int rc = C->work.ensure(3, 1, P->t)) return rc;
-        if (int rc = ensure_win012(*C)) return rc;
+        if (int rc = C->ensure_win012()) return rc;
         if (int rc = C->order_work(C->stream)) return rc;
         FIP_HIP(hipStreamSynchronize(C->stream));
         const size_t ib = image_size * image_size * sizeof(float);

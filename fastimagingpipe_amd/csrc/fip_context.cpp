@@ -78,11 +78,8 @@ void PinnedBuf::release()
 
 // ---------------------------------------------------------------- work
 
-namespace {
-
-constexpr int kWorkBufs = 12;
-
-// Bytes of each work buffer for S snapshots, in the order of Work::bufs().
+// Bytes of each work buffer for S snapshots, in the order rec, bkslot,
+// sorted, hist, bstart, meta, geom, kt, pre, img, work, pmax.
 void work_sizes(int S, size_t stride, const PlanTables& p, size_t out[kWorkBufs])
 {
     const size_t s = (size_t)S;
@@ -101,8 +98,6 @@ void work_sizes(int S, size_t stride, const PlanTables& p, size_t out[kWorkBufs]
                                      pmax_words(S) * sizeof(unsigned)};              // pmax
     std::copy(sizes, sizes + kWorkBufs, out);
 }
-
-}  // namespace
 
 size_t Work::bytes(int S, size_t stride, const PlanTables& p) const
 {
@@ -147,6 +142,24 @@ WorkView Work::view(size_t stride) const
     return v;
 }
 
+WorkView snapshot_view(const WorkView& w, int s, const PlanTables& p)
+{
+    const size_t S = (size_t)s;
+    WorkView v = w;
+    v.rec = w.rec + S * w.stride;
+    v.bkslot = w.bkslot + S * w.stride;
+    v.sorted = w.sorted + S * w.stride;
+    v.hist = w.hist + S * p.G;
+    v.bstart = w.bstart + S * (p.G + 1);
+    v.meta = w.meta + S;
+    v.geom = w.geom + S;
+    v.kt = w.kt + S * (size_t)p.I * (p.M + 1);
+    v.pre = w.pre + S * (size_t)p.I * p.I;
+    v.img = w.img + S * (size_t)p.I * p.I;
+    v.pmax = w.pmax + pmax_index(s, 0, 0);
+    return v;
+}
+
 void Work::release()
 {
     for (DevBuf* b : {&rec, &bkslot, &sorted, &hist, &bstart, &meta, &geom, &kt, &pre, &img, &work, &pmax})
@@ -171,6 +184,8 @@ int Context::get(Context** out)
         auto* c = new Context();
         FIP_HIP(hipGetDevice(&c->device));
         FIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        FIP_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        for (hipEvent_t& e : c->in_ready) FIP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         FIP_HIP(hipEventCreateWithFlags(&c->work_done_, hipEventDisableTiming));
         for (auto& pair : c->ev_) {
             FIP_HIP(hipEventCreate(&pair[0]));
@@ -180,6 +195,22 @@ int Context::get(Context** out)
     }
     *out = ctx;
     return FIP_OK;
+}
+
+void plan_host_tables(size_t image, PlanHostTables& t)
+{
+    const int G = (int)grid_size_for(image);
+    const int M = G / 2;
+    t.twg.assign(G, make_float2(0.0f, 0.0f));
+    t.twm.assign(M, make_float2(0.0f, 0.0f));
+    t.wpost.resize(M);
+    make_stage_twiddles(G, t.twg.data());
+    make_stage_twiddles(M, t.twm.data());
+    for (int k = 0; k < M; ++k) {
+        const double a = 2.0 * M_PI * (double)k / (double)G;
+        t.wpost[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    t.corr = corr_table(image, (size_t)G);
 }
 
 int Context::plan(size_t image, Plan** out)
@@ -197,22 +228,16 @@ int Context::plan(size_t image, Plan** out)
     P->t.I = I;
     P->t.G = G;
     P->t.M = M;
-    std::vector<float2> twg(G), twm(M), wp(M);
-    make_stage_twiddles(G, twg.data());
-    make_stage_twiddles(M, twm.data());
-    for (int k = 0; k < M; ++k) {
-        const double a = 2.0 * M_PI * (double)k / (double)G;
-        wp[k] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    const std::vector<float> corr = corr_table(image, (size_t)G);
-    FIP_HIP(P->tw_g.ensure(twg.size() * sizeof(float2)));
-    FIP_HIP(P->tw_m.ensure(twm.size() * sizeof(float2)));
-    FIP_HIP(P->wpost.ensure(wp.size() * sizeof(float2)));
-    FIP_HIP(P->corr.ensure(corr.size() * sizeof(float)));
-    FIP_HIP(hipMemcpy(P->tw_g.p, twg.data(), twg.size() * sizeof(float2), hipMemcpyHostToDevice));
-    FIP_HIP(hipMemcpy(P->tw_m.p, twm.data(), twm.size() * sizeof(float2), hipMemcpyHostToDevice));
-    FIP_HIP(hipMemcpy(P->wpost.p, wp.data(), wp.size() * sizeof(float2), hipMemcpyHostToDevice));
-    FIP_HIP(hipMemcpy(P->corr.p, corr.data(), corr.size() * sizeof(float), hipMemcpyHostToDevice));
+    PlanHostTables h;
+    plan_host_tables(image, h);
+    FIP_HIP(P->tw_g.ensure(h.twg.size() * sizeof(float2)));
+    FIP_HIP(P->tw_m.ensure(h.twm.size() * sizeof(float2)));
+    FIP_HIP(P->wpost.ensure(h.wpost.size() * sizeof(float2)));
+    FIP_HIP(P->corr.ensure(h.corr.size() * sizeof(float)));
+    FIP_HIP(hipMemcpy(P->tw_g.p, h.twg.data(), h.twg.size() * sizeof(float2), hipMemcpyHostToDevice));
+    FIP_HIP(hipMemcpy(P->tw_m.p, h.twm.data(), h.twm.size() * sizeof(float2), hipMemcpyHostToDevice));
+    FIP_HIP(hipMemcpy(P->wpost.p, h.wpost.data(), h.wpost.size() * sizeof(float2), hipMemcpyHostToDevice));
+    FIP_HIP(hipMemcpy(P->corr.p, h.corr.data(), h.corr.size() * sizeof(float), hipMemcpyHostToDevice));
     P->t.tw_g = P->tw_g.as<float2>();
     P->t.tw_m = P->tw_m.as<float2>();
     P->t.wpost = P->wpost.as<float2>();
@@ -222,10 +247,8 @@ int Context::plan(size_t image, Plan** out)
     return FIP_OK;
 }
 
-Tuning Context::tuning(size_t image) const
+Tuning default_tuning(size_t image)
 {
-    auto it = tunings_.find(image);
-    if (it != tunings_.end()) return it->second;
     Tuning t;
     t.stage1_records = 4;         // gridding records per thread in registers
     // Rows per C2R workgroup: four rows of up to 768 points, two of 1536 and
@@ -238,9 +261,26 @@ Tuning Context::tuning(size_t image) const
     return t;
 }
 
+Tuning Context::tuning(size_t image) const
+{
+    auto it = tunings_.find(image);
+    if (it != tunings_.end()) return it->second;
+    return default_tuning(image);
+}
+
 int Context::order_work(hipStream_t st)
 {
     if (work_stream_ && work_stream_ != st) FIP_HIP(hipStreamWaitEvent(st, work_done_, 0));
+    return FIP_OK;
+}
+
+int Context::ensure_win012()
+{
+    if (win012_ && win.p) return FIP_OK;
+    static const int w[3] = {0, 1, 2};
+    FIP_HIP(win.ensure(sizeof(w)));
+    FIP_HIP(hipMemcpy(win.p, w, sizeof(w), hipMemcpyHostToDevice));
+    win012_ = true;
     return FIP_OK;
 }
 
@@ -283,21 +323,24 @@ int Context::collect_stages()
 
 // -------------------------------------------------------- launch sequence
 
-#define FIP_STAGE(C, st, s, call)                      \
-    do {                                               \
-        int rc_ = (C).begin_stage((st), (s));          \
-        if (rc_) return rc_;                           \
-        hipError_t e_ = (call);                        \
+// Stage launch with optional profiling events (Cp may be null).
+#define FIP_STAGE(Cp, st, s, call)                                 \
+    do {                                                           \
+        if (Cp) {                                                  \
+            int rc_ = (Cp)->begin_stage((st), (s));                \
+            if (rc_) return rc_;                                   \
+        }                                                          \
+        hipError_t e_ = (call);                                    \
         if (e_ != hipSuccess) return hip_error(e_, stage_name(st)); \
-        rc_ = (C).end_stage((st), (s));                \
-        if (rc_) return rc_;                           \
+        if (Cp) {                                                  \
+            int rc_ = (Cp)->end_stage((st), (s));                  \
+            if (rc_) return rc_;                                   \
+        }                                                          \
     } while (0)
 
-int run_imaging(Context& C, const Plan& P, Work& W, const InputView& in, int S, int nmax, float cell,
-                const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st)
+int imaging_sequence(Context* C, const PlanTables& p, const WorkView& w, const InputView& in, int S, int nmax,
+                     float cell, const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st)
 {
-    const PlanTables& p = P.t;
-    const WorkView w = W.view(in.stride);
     // r1r2_scale = cell * g (FIPkernels.cu:584), kept in FP64.
     const double scale = (double)cell * (double)p.G;
     FIP_STAGE(C, kStReset, st, launch_reset(in, w, p, S, cell, norm_dev, norm_const, st));
@@ -310,19 +353,30 @@ int run_imaging(Context& C, const Plan& P, Work& W, const InputView& in, int S, 
     return FIP_OK;
 }
 
+int trigger_sequence(Context* C, const PlanTables& p, const WorkView& w, const int* win_dev, int nwin, int tile,
+                     float* out_dev, hipStream_t st)
+{
+    FIP_STAGE(C, kStTrigger, st, launch_trigger(w, p, win_dev, nwin, tile, out_dev, st));
+    return FIP_OK;
+}
+
+int run_imaging(Context& C, const Plan& P, Work& W, const InputView& in, int S, int nmax, float cell,
+                const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st)
+{
+    return imaging_sequence(&C, P.t, W.view(in.stride), in, S, nmax, cell, norm_dev, norm_const, tu, st);
+}
+
 int run_trigger(Context& C, const Plan& P, Work& W, size_t stride, const int* win_dev, int nwin, int tile,
                 float* out_dev, hipStream_t st)
 {
-    const WorkView w = W.view(stride);
-    FIP_STAGE(C, kStTrigger, st, launch_trigger(w, P.t, win_dev, nwin, tile, out_dev, st));
-    return FIP_OK;
+    return trigger_sequence(&C, P.t, W.view(stride), win_dev, nwin, tile, out_dev, st);
 }
 
 int run_trigger_run(Context& C, const Plan& P, Work& W, size_t stride, int s0, int nwin, int tile, float* out_dev,
                     hipStream_t st)
 {
     const WorkView w = W.view(stride);
-    FIP_STAGE(C, kStTrigger, st, launch_trigger_run(w, P.t, s0, nwin, tile, out_dev, st));
+    FIP_STAGE(&C, kStTrigger, st, launch_trigger_run(w, P.t, s0, nwin, tile, out_dev, st));
     return FIP_OK;
 }
 

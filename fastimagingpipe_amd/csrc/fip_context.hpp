@@ -45,6 +45,23 @@ struct Plan {
     DevBuf tw_g, tw_m, wpost, corr;
 };
 
+// Host copies of one image size's plan tables (FP64-computed, FP32-stored):
+// stage twiddles of the length-G and length-M transforms, the C2R split
+// factors and the per-axis grid correction.
+struct PlanHostTables {
+    std::vector<float2> twg, twm, wpost;
+    std::vector<float> corr;
+};
+void plan_host_tables(size_t image, PlanHostTables& t);
+
+// Launch parameters used until the autotuner or fip_set_tuning picks others.
+Tuning default_tuning(size_t image);
+
+// Work buffers in the order rec, bkslot, sorted, hist, bstart, meta, geom,
+// kt, pre, img, work, pmax: their byte sizes for S snapshots.
+constexpr int kWorkBufs = 12;
+void work_sizes(int S, size_t stride, const PlanTables& p, size_t out[kWorkBufs]);
+
 // Work buffers sized for S snapshots of one image size and stride.
 struct Work {
     DevBuf rec, bkslot, sorted, hist, bstart, meta, geom, kt, pre, img, work, pmax;
@@ -80,12 +97,25 @@ public:
     Work work;
     DevBuf in_vr, in_vi, in_b, in_v, out, win, nvis, norm;
     PinnedBuf staging;
+    // Host drop-in: per-snapshot input regions on the device, filled on
+    // copy_stream; in_ready[s] marks snapshot s's region as copied.
+    DevBuf in_snap;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t in_ready[3] = {};
+    // Preprocessing (projection, collapse and the raw-input chain): device
+    // inputs of the host-array entry points, per-block moment partials,
+    // per-snapshot projection parameters, and pinned memory for the partials
+    // read back and the parameters sent.
+    DevBuf pp_uvw, pp_freq, pp_in, pp_part, pp_par;
+    PinnedBuf pp_host;
 
     // Every entry point that enqueues work on `work` from some stream calls
     // order_work(st) before and mark_work(st) after, so uses of the shared
     // buffers from different streams (the context's own and callers') run
     // one after another in call order.
     int order_work(hipStream_t st);
+    // C.win holds the window {0, 1, 2} (uploaded once).
+    int ensure_win012();
     int mark_work(hipStream_t st);
 
 private:
@@ -95,7 +125,18 @@ private:
     bool ev_used_[kStCount] = {};
     hipEvent_t work_done_ = nullptr;     // recorded after the last enqueue on `work`
     hipStream_t work_stream_ = nullptr;  // the stream of that enqueue
+    bool win012_ = false;
 };
+
+// The work buffers of snapshot s of `w` as a one-snapshot view.
+WorkView snapshot_view(const WorkView& w, int s, const PlanTables& p);
+
+// Enqueue reset..splat / the trigger on explicit tables and buffers; C (may
+// be null) only adds the profiling events.
+int imaging_sequence(Context* C, const PlanTables& p, const WorkView& w, const InputView& in, int S, int nmax,
+                     float cell, const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st);
+int trigger_sequence(Context* C, const PlanTables& p, const WorkView& w, const int* win_dev, int nwin, int tile,
+                     float* out_dev, hipStream_t st);
 
 // Enqueue reset..splat for S snapshots described by `in` into `w`.
 int run_imaging(Context& C, const Plan& P, Work& W, const InputView& in, int S, int nmax, float cell,

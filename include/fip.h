@@ -146,6 +146,39 @@ int fip_project(const double *uvw, size_t nrows, const double *freqs,
                 size_t nchan, float *bin, float *vin);
 
 /*
+ * Raw per-snapshot inputs in device memory for fip_pipe_raw_device (the
+ * device-resident form of SVDPreprocessGpu.h:10-32's HostMeasurementSetData,
+ * one block per snapshot).  Sample n of a snapshot is row n % nrows of
+ * channel n / nrows (channel-major, SVD_MFS.py:48-52, 94-95).
+ */
+typedef struct fip_raw_view {
+    const double *uvw;            /* [nsnap][nrows][3] metres                */
+    const double *freqs;          /* [nchan] Hz, shared by every snapshot    */
+    const float *xx_re, *xx_im;   /* [nsnap][nchan*nrows] polarisation 0     */
+    const float *yy_re, *yy_im;   /* [nsnap][nchan*nrows] polarisation 3     */
+    const unsigned char *flag0;   /* [nsnap][nchan*nrows] 0/1                */
+    const unsigned char *flag3;
+    const float *w0, *w3;         /* [nsnap][nchan*nrows] WEIGHT_SPECTRUM    */
+    size_t nrows, nchan, nsnap;
+} fip_raw_view;
+
+/*
+ * The whole chain on the device, from raw visibilities to trigger: the
+ * weighted XX/YY collapse (SVDPreprocessKernels.cu:208-237), the SVD-plane
+ * projection B = D.[V0 V1] (SVD_MFS.py:38-75, 98-110;
+ * SVDPreprocessKernels.cu:255-424; FP64 moments in one pass, the 3x3
+ * eigen-solve on the host after one small read-back), then FIpipe's imaging
+ * and trigger (FIPkernels.cu:554-862) of snapshots 0..2 with N = nchan*nrows
+ * samples each.  result: device (image/unit)^2 floats.  bin_out ([3][N][2])
+ * and vin_out ([3][9]) receive B and V when not NULL (device).  Returns once
+ * the work after the eigen-solve is enqueued on `stream` (NULL: the
+ * library's stream); the result is ready when the stream is.
+ */
+int fip_pipe_raw_device(const fip_raw_view *raw, size_t image_size,
+                        float cell_size, size_t unit_size, float *result,
+                        float *bin_out, float *vin_out, void *stream);
+
+/*
  * Weighted XX/YY collapse (SVD_MFS.py:77-96; SVDPreprocessKernels.cu:208-237):
  * (w0*XX + w3*YY)/(w0+w3) with w = weight*!flag, 0 where w0+w3 <= 0.
  * Host arrays of n samples.

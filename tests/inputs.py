@@ -97,3 +97,52 @@ def from_bf16_bits(b):
 
 def triggered(tlisi, tau):
     return set(zip(*np.nonzero(np.asarray(tlisi) < tau)))
+
+
+def raw_snapshots(seed: int, nant: int, radius_m: float, nsnap: int, freqs, image: int, cell: float,
+                  nsrc: int = 30, noise: float = 0.05, flag_frac: float = 0.1, dec_deg: float = -30.0,
+                  lat_deg: float = -26.7, dt_s: float = 1.0):
+    """Raw per-snapshot inputs of the preprocessing chain (numpy only): uvw of
+    every antenna pair of a log-radial array under earth rotation, and XX/YY
+    from an FP64 DFT of point sources over D = uvw nu/c including the w term,
+    with noise, random flags and weights.  Samples are channel-major
+    (n = row + chan*nrows).  Returns uvw [S][nrows][3] (float64), freqs,
+    xx, yy [S][ns] complex64, flag0, flag3 [S][ns] uint8, w0, w3 [S][ns] f32."""
+    rng = np.random.default_rng(seed)
+    freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+    r = np.exp(rng.uniform(np.log(20.0), np.log(radius_m), nant))
+    az = rng.uniform(0, 2 * np.pi, nant)
+    east, north = r * np.cos(az), r * np.sin(az)
+    lat, dec = np.radians(lat_deg), np.radians(dec_deg)
+    X, Y, Z = -np.sin(lat) * north, east, np.cos(lat) * north
+    i, j = np.triu_indices(nant, 1)
+    bx, by, bz = X[j] - X[i], Y[j] - Y[i], Z[j] - Z[i]
+    nrows = bx.size
+    fov = 0.3 * image * cell
+    l = rng.uniform(-fov, fov, nsrc)
+    m = rng.uniform(-fov, fov, nsrc)
+    n1 = np.sqrt(1.0 - l * l - m * m) - 1.0
+    flux = rng.uniform(0.05, 1.0, nsrc)
+    c = 299792458.0
+    ns = nrows * freqs.size
+    uvw = np.zeros((nsnap, nrows, 3))
+    xx = np.zeros((nsnap, ns), np.complex64)
+    yy = np.zeros((nsnap, ns), np.complex64)
+    for s in range(nsnap):
+        H = 2 * np.pi * s * dt_s / 86164.0
+        u = np.sin(H) * bx + np.cos(H) * by
+        v = -np.sin(dec) * np.cos(H) * bx + np.sin(dec) * np.sin(H) * by + np.cos(dec) * bz
+        w = np.cos(dec) * np.cos(H) * bx - np.cos(dec) * np.sin(H) * by + np.sin(dec) * bz
+        uvw[s] = np.stack([u, v, w], axis=1)
+        vis = np.zeros(ns, np.complex128)
+        for ch, f in enumerate(freqs):
+            D = uvw[s] * (f / c)
+            ph = -2j * np.pi * (D[:, :1] * l[None, :] + D[:, 1:2] * m[None, :] + D[:, 2:3] * n1[None, :])
+            vis[ch * nrows:(ch + 1) * nrows] = (flux[None, :] * np.exp(ph)).sum(axis=1)
+        xx[s] = vis + noise * (rng.standard_normal(ns) + 1j * rng.standard_normal(ns))
+        yy[s] = vis + noise * (rng.standard_normal(ns) + 1j * rng.standard_normal(ns))
+    flag0 = (rng.uniform(size=(nsnap, ns)) < flag_frac).astype(np.uint8)
+    flag3 = (rng.uniform(size=(nsnap, ns)) < flag_frac).astype(np.uint8)
+    w0 = rng.uniform(0.5, 2.0, (nsnap, ns)).astype(np.float32)
+    w3 = rng.uniform(0.5, 2.0, (nsnap, ns)).astype(np.float32)
+    return uvw, freqs, xx, yy, flag0, flag3, w0, w3

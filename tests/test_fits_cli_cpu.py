@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CLI = os.path.join(ROOT, "sharedlibrary_gpu")
+CLI = os.environ.get("FIP_CLI") or os.path.join(ROOT, "sharedlibrary_gpu")
 
 
 @pytest.fixture(scope="module")
