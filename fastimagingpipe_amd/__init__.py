@@ -14,7 +14,7 @@ import numpy as np
 
 __all__ = [
     "FipError", "lib", "library_path", "grid_size", "fipipe", "fipipe_lowp", "image",
-    "trigger", "pipe_device", "pipe_raw_device", "batch_device", "batch_host", "project", "collapse", "ingest_table",
+    "trigger", "pipe_device", "pipe_device_noalloc", "pipe_raw_device", "batch_device", "batch_host", "project", "collapse", "ingest_table",
     "synth",
     "set_profiling", "stage_times", "reset_stage_times", "autotune", "set_tuning",
     "debug_fft", "fits_read", "fits_write", "fits_write3", "Server", "STATUS",
@@ -69,6 +69,7 @@ _sig("fip_device_count", [])
 _sig("fip_grid_size", [_sz], _sz)
 _sig("fip_pipe", [_vp, _vp, _vp, _vp, _vp, _sz, _sz, _sz, _f, _sz])
 _sig("fip_pipe_device", [_vp, _vp, _vp, _vp, _vp, _sz, _sz, _sz, _f, _sz, _vp])
+_sig("fip_pipe_device_noalloc", [_vp, _vp, _vp, _vp, _vp, _sz, _sz, _sz, _f, _sz, _vp])
 _sig("fip_pipe_lowp", [_vp, _vp, _i, _vp, _vp, _vp, _sz, _sz, _sz, _f, _sz])
 _sig("fip_image", [_vp, _vp, _vp, _vp, _sz, _sz, _sz, _f, _vp, _vp, _vp])
 _sig("fip_trigger", [_vp, _vp, _vp, _vp, _sz, _sz, _vp])
@@ -215,6 +216,15 @@ def pipe_device(vr_ptr: int, vi_ptr: int, b_ptr: int, v_ptr: int, out_ptr: int,
     """fip_pipe_device on raw device pointers (e.g. torch ``data_ptr()``)."""
     _check(lib.fip_pipe_device(vr_ptr, vi_ptr, b_ptr, v_ptr, out_ptr, num_baselines, image_size,
                                num_snapshots, cell_size, unit_size, stream or None))
+
+
+def pipe_device_noalloc(vr_ptr: int, vi_ptr: int, b_ptr: int, v_ptr: int, out_ptr: int,
+                        num_baselines: int, image_size: int, num_snapshots: int, cell_size: float,
+                        unit_size: int, stream: int = 0) -> None:
+    """fip_pipe_device_noalloc: as pipe_device, with every intermediate in the
+    library's static device array (no allocation; image sizes up to 1024)."""
+    _check(lib.fip_pipe_device_noalloc(vr_ptr, vi_ptr, b_ptr, v_ptr, out_ptr, num_baselines, image_size,
+                                       num_snapshots, cell_size, unit_size, stream or None))
 
 
 def batch_device(vr_ptr: int, vi_ptr: int, vis_type: int, b_ptr: int, v_ptr: int, nvis,

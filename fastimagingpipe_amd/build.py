@@ -33,6 +33,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     "fip_kernels.hip",
     "fip_preprocess.hip",
+    "fip_noalloc.hip",
     "fip_plan.cpp",
     "fip_context.cpp",
     "fip_api.cpp",

@@ -75,6 +75,25 @@ int fip_pipe_device(const float *vis_re, const float *vis_im, const float *bin,
                     size_t unit_size, void *stream);
 
 /*
+ * fip_pipe_device that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * stream capture, serving loops or harnesses that forbid hipMalloc).  Every
+ * intermediate buffer and plan table lives in one static 64 MiB device array
+ * per GPU, held until the process ends; the call makes no hipMalloc,
+ * hipFree or other allocating call.  Image sizes 64..1024 whose work fits
+ * the array (1024^2 with up to 130816 baselines); larger shapes return
+ * FIP_ERR_SIZE.  The first call on a device writes the plan tables into the
+ * array with synchronous copies; later calls only enqueue kernels on
+ * `stream` (NULL: the default stream).  All calls on one GPU share the
+ * array: calls that overlap in time on one GPU race.
+ */
+int fip_pipe_device_noalloc(const float *vis_re, const float *vis_im,
+                            const float *bin, const float *vin, float *result,
+                            size_t num_baselines, size_t image_size,
+                            size_t num_snapshots, float cell_size,
+                            size_t unit_size, void *stream);
+
+/*
  * Precision form of fip_pipe: visibilities stored as bfloat16 or IEEE half
  * (vis_type FIP_VIS_BF16 / FIP_VIS_F16; 2-byte elements), converted to FP32
  * on load; gridding accumulates contributions scaled by a power of two per
