@@ -137,20 +137,6 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
     return x * per + min(x, rem) + k;
 }
 
-// Hardware block bid runs on XCD bid % 8.  Map it so that XCD x takes
-// logical blocks in runs of kColChunk consecutive ids, the runs of the eight
-// XCDs interleaved: k-th block of XCD x -> (k / K) * 8K + x K + k % K.  A
-// bijection on [0, nb): blocks past the last whole group of 8K keep their id.
-constexpr int kColChunk = 24;
-__device__ __forceinline__ int xcd_chunk_remap(int bid, int nb)
-{
-    constexpr int nx = 8, K = kColChunk;
-    const int full = nb / (nx * K) * (nx * K);
-    if (bid >= full) return bid;
-    const int x = bid % nx, k = bid / nx;
-    return (k / K) * (nx * K) + x * K + k % K;
-}
-
 __device__ __forceinline__ long long to_fixed(float x)
 {
     return (long long)rintf(x);   // |x| < 2^62 by construction; exact for |x| >= 2^24
@@ -451,38 +437,21 @@ __device__ __forceinline__ int tap_row(const ColRec& c, int t1, int M)
     return c.mirror ? ((t1 == -M) ? -M : -t1) : t1;
 }
 
-// One LDS cell holds the (re, im) integer sums of a row packed in a 64-bit
-// word, re + im * 2^32, accumulated with one 64-bit integer atomic per tap.
-// The sum of the packed signed contributions is exactly Re + Im * 2^32 (no
-// wrap: |Im| < 2^31), and since |Re| < 2^31 the low word read as int32 is Re
-// and the rest is Im: one atomic per tap instead of two, with the same
-// exact, order-independent integer sums.
-__device__ __forceinline__ unsigned long long pack_re_im(int re, int im)
-{
-    return (unsigned long long)((long long)re + (long long)im * 4294967296ll);
-}
-
-__device__ __forceinline__ int2 unpack_re_im(unsigned long long v)
-{
-    const int re = (int)(unsigned)(v & 0xffffffffull);
-    const long long im = ((long long)v - (long long)re) / 4294967296ll;
-    return make_int2(re, (int)im);
-}
-
 // Row taps of one prepared record, accumulated as integers into the LDS
-// column (packed re/im per row; row r at padded index of r mod G), each
-// scaled by its block's power of two.
-__device__ __forceinline__ void grid_taps(const ColRec& c, int M, int G, int rbs, const float* sc,
-                                          unsigned long long* col)
+// column (re, im interleaved; row r at padded index of r mod G), each scaled
+// by its block's power of two.
+__device__ __forceinline__ void grid_taps(const ColRec& c, int M, int G, int rbs, const float* sc, int* col)
 {
     const int lo1 = max(c.b1 - 4 + (c.f1 > 0.0f ? 1 : 0), -M);
     const int hi1 = min(c.b1 + 4, M - 1);
     for (int t1 = lo1; t1 <= hi1; ++t1) {
         const int row = tap_row(c, t1, M);
         const int idx = row < 0 ? row + G : row;
+        const int q = 2 * lds_pad(idx);
         const float w1 = es_f(((float)(t1 - c.b1) - c.f1) * 0.25f);
         const float k = sc[idx / rbs];
-        atomicAdd(&col[lds_pad(idx)], pack_re_im((int)rintf((c.vr * w1) * k), (int)rintf((c.vi * w1) * k)));
+        atomicAdd(&col[q], (int)rintf((c.vr * w1) * k));
+        atomicAdd(&col[q + 1], (int)rintf((c.vi * w1) * k));
     }
 }
 
@@ -582,7 +551,7 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
     };
     const bool resident = ntot <= KR * NT;
     if (resident) load(0);
-    unsigned long long* col = reinterpret_cast<unsigned long long*>(fb);
+    int* col = reinterpret_cast<int*>(fb);
     for (int i = t; i < G; i += NT) fb[lds_pad(i)] = make_float2(0.0f, 0.0f);   // integer zero too
     for (int i = t; i < kScaleBlocks; i += NT) mass[i] = 0ull;
     __syncthreads();
@@ -615,9 +584,9 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
     for (int i = rlo + t; i <= rhi; i += NT) {
         const int idx = i < 0 ? i + G : i;
         const int q = lds_pad(idx);
-        const int2 v = unpack_re_im(col[q]);
+        const int re = col[2 * q], im = col[2 * q + 1];
         const float k = isc[idx / RBS];
-        fb[q] = make_float2((float)v.x * k, (float)v.y * k);
+        fb[q] = make_float2((float)re * k, (float)im * k);
     }
     __syncthreads();
     fft_inverse<G>(fb, tw, t);
@@ -645,13 +614,9 @@ __global__ __launch_bounds__(stage1_threads<G>(), (stage1_threads<G>() >= 1024 ?
     __shared__ float sc[kScaleBlocks], isc[kScaleBlocks];
     // Column-major over snapshots: the heavily populated low columns of
     // every snapshot are dispatched first, so none of them ends the launch.
-    // Runs of kColChunk consecutive (column, snapshot) items go to one XCD,
-    // so neighbouring columns, which share 8 of their 9 record buckets,
-    // read them through one L2; the runs still interleave over the XCDs.
     const int S = (int)(gridDim.x / (G / 2 + 1));
-    const int item = xcd_chunk_remap((int)blockIdx.x, (int)gridDim.x);
-    const int s = item % S;
-    const int c = item / S;
+    const int s = (int)(blockIdx.x % S);
+    const int c = (int)(blockIdx.x / S);
     // The bucket ranges do not depend on the metadata: both loads are in
     // flight together.
     const int4 rg = column_ranges<G>(bstart + (size_t)s * (G + 1), c);
