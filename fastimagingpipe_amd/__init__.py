@@ -414,7 +414,8 @@ class Server:
 
     def submit(self, sid: int, vis_re, vis_im, bin_, vin, arrival_s: float = -1.0) -> bool:
         """arrival_s on time.monotonic()'s clock (negative: now).  Returns False when the
-        snapshot is refused (no free input slot)."""
+        snapshot is refused (its inputs do not fit the free part of the input pool
+        that the HBM budget leaves)."""
         vt = self._streams[sid].get("vis_type", VIS_F32)
         if vt == VIS_F32:
             vr, vi = _c(vis_re), _c(vis_im)

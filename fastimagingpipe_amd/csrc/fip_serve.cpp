@@ -1,15 +1,23 @@
 // Serving form: streaming imaging + trigger for several sub-array streams.
 //
 // Each stream owns a HIP stream (default priority), single-snapshot work
-// buffers, a device ring of `ring` image slots and a queue of input slots.
-// A submitted snapshot is copied to a free input slot, imaged into the next
-// ring slot and, from the third snapshot on, triggered against the two
-// previous ring slots (the window of FIPkernels.cu:806-851); the tLISI is
-// copied to pinned memory and an event recorded behind it; a completion
-// thread polls the events of the snapshots in flight and stamps each
-// completion time, so no host callback ever stalls a stream.  Admission: device memory for all streams must fit the HBM
-// budget at creation; a snapshot is refused when all of its stream's input
-// slots are still in flight.
+// buffers, a device ring of `ring` image slots and its window table.  A
+// submitted snapshot reserves room for its inputs (and its pinned tLISI) in
+// an input pool shared by all streams, is copied there through the pool's
+// pinned mirror, imaged into the next ring slot and, from the third snapshot
+// on, triggered against the two previous ring slots (the window of
+// FIPkernels.cu:806-851); the tLISI is copied to pinned memory and an event
+// recorded behind it.  A completion thread polls the events of the
+// snapshots in flight, stamps each completion time and releases the
+// reservation, so no host callback ever stalls a stream.
+//
+// Admission by HBM budget: at creation every stream's fixed device state
+// (work buffers, ring, window table) plus one snapshot of the largest
+// stream's inputs must fit the budget (else FIP_ERR_BUDGET); what remains,
+// up to `ring` snapshots per stream, becomes the input pool.  Each arrival is
+// admitted only if its reservation fits the pool's free space at that
+// moment, so sub-arrays of different sizes share the budget snapshot by
+// snapshot; a refused arrival returns FIP_ERR_BUDGET and is counted.
 #include "fip.h"
 
 #include "fip_context.hpp"
@@ -21,6 +29,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -35,7 +44,41 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-struct InSlot;
+size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+size_t esize(int vt) { return vt == FIP_VIS_F32 ? 4 : 2; }
+
+// Byte layout of one snapshot's reservation: vis_re | vis_im | bin | vin |
+// tLISI (the last only used in the pinned mirror).
+struct InLayout {
+    size_t vi, b, v, res, bytes;
+};
+
+InLayout in_layout(const fip_stream_config& c)
+{
+    const size_t N = c.num_baselines, un = c.image_size / c.unit_size;
+    InLayout L;
+    L.vi = al256(N * esize(c.vis_type));
+    L.b = L.vi + al256(N * esize(c.vis_type));
+    L.v = L.b + al256(N * 2 * sizeof(float));
+    L.res = L.v + al256(9 * sizeof(float));
+    L.bytes = L.res + al256(un * un * sizeof(float));
+    return L;
+}
+
+// Fixed device bytes of one stream: single-snapshot work buffers, the ring
+// of images with their metadata and max slots, window table and tLISI.
+size_t stream_fixed_bytes(const fip_stream_config& c, int ring)
+{
+    const size_t I = c.image_size, N = c.num_baselines;
+    const size_t G = grid_size_for(I), M = G / 2;
+    const size_t un = I / c.unit_size;
+    const size_t work = N * sizeof(VisRec) * 2 + N * sizeof(int2) + G * 4 + (G + 1) * 4 + I * (M + 1) * 8 + I * I * 4;
+    const size_t ringb = (size_t)ring * (I * I * 4 + sizeof(SnapMeta) + sizeof(SnapGeom)) +
+                         pmax_words(ring) * sizeof(unsigned) + 3 * sizeof(int) * (size_t)ring;
+    return work + ringb + un * un * 4 + 256;
+}
+
 }  // namespace
 
 struct fip_server;
@@ -43,13 +86,10 @@ struct fip_server;
 namespace {
 
 struct InSlot {
-    DevBuf vr, vi, b, v;
-    PinnedBuf host;        // staging for the four input arrays
-    PinnedBuf result;      // pinned tLISI of this snapshot
-    std::atomic<int> busy{0};
+    size_t off = 0, bytes = 0;   // reservation in the input pool
     hipEvent_t done = nullptr;   // recorded after the tLISI copy
     double arrival = 0.0;
-    long window = -1;      // window index produced by this snapshot, -1 if none
+    long window = -1;            // window index produced by this snapshot, -1 if none
     int stream_id = 0;
     fip_server* srv = nullptr;
 };
@@ -62,12 +102,53 @@ struct SubStream {
     DevBuf rec, bkslot, sorted, hist, bstart, kt, pre, work;   // one snapshot
     DevBuf img, meta, geom, pmax;                              // ring
     DevBuf win, out;
-    std::vector<std::unique_ptr<InSlot>> slots;
+    InLayout lay{};
     long submitted = 0;
     std::vector<float> latest;
     long latest_window = -1;
-    size_t in_bytes = 0;
     int G = 0, M = 0;
+};
+
+// First-fit allocator over [0, size) with coalescing frees.
+class PoolAlloc {
+public:
+    void reset(size_t size)
+    {
+        free_.clear();
+        if (size) free_[0] = size;
+    }
+    bool take(size_t bytes, size_t* off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes;
+            const size_t next = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[next] = rest;
+            return true;
+        }
+        return false;
+    }
+    void give(size_t off, size_t bytes)
+    {
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+private:
+    std::map<size_t, size_t> free_;
 };
 
 }  // namespace
@@ -78,6 +159,14 @@ struct fip_server {
     std::mutex mu;                 // guards latencies / latest / counters
     std::vector<double> lat_ms;
     long processed = 0, refused = 0, inflight = 0;
+    // input pool (device) and its pinned mirror, with the reservations
+    DevBuf pool;
+    PinnedBuf hpool;
+    size_t pool_bytes = 0;
+    std::mutex amu;                // guards alloc and idle
+    PoolAlloc alloc;
+    std::vector<std::unique_ptr<InSlot>> slots;   // every slot object ever made
+    std::vector<InSlot*> idle;                    // ready for reuse (event created)
     // completion tracking
     std::mutex pmu;
     std::condition_variable pcv;
@@ -100,12 +189,15 @@ void on_complete(InSlot* sl, double t)
             SubStream& ss = *srv->streams[sl->stream_id];
             const size_t un = ss.cfg.image_size / ss.cfg.unit_size;
             if (sl->window > ss.latest_window) {
-                std::memcpy(ss.latest.data(), sl->result.p, un * un * sizeof(float));
+                std::memcpy(ss.latest.data(), static_cast<char*>(srv->hpool.p) + sl->off + ss.lay.res,
+                            un * un * sizeof(float));
                 ss.latest_window = sl->window;
             }
         }
     }
-    sl->busy.store(0);
+    std::lock_guard<std::mutex> lk(srv->amu);
+    srv->alloc.give(sl->off, sl->bytes);
+    srv->idle.push_back(sl);
 }
 
 // Completion thread: polls the events of the snapshots in flight (any
@@ -137,20 +229,6 @@ void poll_loop(fip_server* srv)
     }
 }
 
-size_t esize(int vt) { return vt == FIP_VIS_F32 ? 4 : 2; }
-
-// Device bytes of one stream with q input slots.
-size_t stream_bytes(const fip_stream_config& c, int ring, int q)
-{
-    const size_t I = c.image_size, N = c.num_baselines;
-    const size_t G = grid_size_for(I), M = G / 2;
-    const size_t un = I / c.unit_size;
-    const size_t work = N * sizeof(VisRec) * 2 + N * sizeof(int2) + G * 4 + (G + 1) * 4 + I * (M + 1) * 8 + I * I * 4;
-    const size_t ringb = (size_t)ring * (I * I * 4 + sizeof(SnapMeta) + sizeof(SnapGeom)) + pmax_words(ring) * sizeof(unsigned);
-    const size_t in = 2 * N * esize(c.vis_type) + N * 8 + 36;
-    return work + ringb + un * un * 4 + 256 + (size_t)q * in;
-}
-
 }  // namespace
 
 extern "C" {
@@ -168,45 +246,49 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
             if (c.vis_type != FIP_VIS_F32 && c.vis_type != FIP_VIS_BF16 && c.vis_type != FIP_VIS_F16)
                 return set_error(FIP_ERR_ARG, "bad vis_type");
         }
-        // Admission at creation: every stream must fit with one input slot;
-        // the remaining budget is shared out as extra input slots (up to 8).
-        size_t base = 0;
-        for (int k = 0; k < nstreams; ++k) base += stream_bytes(streams[k], ring, 1);
-        if (base > hbm_budget_bytes)
+        // Admission at creation: the fixed state of every stream plus one
+        // snapshot of the largest stream must fit; the rest of the budget, up
+        // to `ring` snapshots per stream, is the shared input pool.
+        size_t base = 0, largest = 0, want = 0;
+        for (int k = 0; k < nstreams; ++k) {
+            base += stream_fixed_bytes(streams[k], ring);
+            const size_t r = in_layout(streams[k]).bytes;
+            largest = std::max(largest, r);
+            want += (size_t)ring * r;
+        }
+        if (base + largest > hbm_budget_bytes)
             return set_error(FIP_ERR_BUDGET, "HBM budget " + std::to_string(hbm_budget_bytes) + " below the " +
-                                                 std::to_string(base) + " bytes one snapshot per stream needs");
+                                                 std::to_string(base + largest) +
+                                                 " bytes the streams and one snapshot need");
         Context* C = nullptr;
         if (int rc = Context::get(&C)) return rc;
         std::lock_guard<std::recursive_mutex> lk(C->mu);
         auto* srv = new fip_server();
         srv->ring = ring;
+        srv->pool_bytes = std::min(want, (hbm_budget_bytes - base) / 256 * 256);
+        auto fail = [&](int rc) {
+            fip_server_destroy(srv);
+            return rc;
+        };
+        hipError_t e = srv->pool.ensure(srv->pool_bytes);
+        if (e != hipSuccess) return fail(hip_error(e, "hipMalloc (server input pool)"));
+        e = srv->hpool.ensure(srv->pool_bytes);
+        if (e != hipSuccess) return fail(hip_error(e, "hipHostMalloc (server input pool)"));
+        srv->alloc.reset(srv->pool_bytes);
         srv->poller = std::thread(poll_loop, srv);
-        size_t spare = hbm_budget_bytes - base;
         for (int k = 0; k < nstreams; ++k) {
             auto ss = std::make_unique<SubStream>();
             ss->cfg = streams[k];
-            if (int rc = C->plan(ss->cfg.image_size, &ss->plan)) {
-                fip_server_destroy(srv);
-                return rc;
-            }
+            ss->lay = in_layout(streams[k]);
+            if (int rc = C->plan(ss->cfg.image_size, &ss->plan)) return fail(rc);
             ss->tu = C->tuning(ss->cfg.image_size);
             ss->G = ss->plan->t.G;
             ss->M = ss->plan->t.M;
             const size_t I = ss->cfg.image_size, N = ss->cfg.num_baselines;
             const size_t un = I / ss->cfg.unit_size;
-            ss->in_bytes = 2 * N * esize(ss->cfg.vis_type) + N * 8 + 36;
-            int q = 1;
-            while (q < 8 && spare >= ss->in_bytes) {
-                spare -= ss->in_bytes;
-                ++q;
-            }
             SubStream* ssp = ss.get();
             srv->streams.push_back(std::move(ss));
-            auto fail = [&](int rc) {
-                fip_server_destroy(srv);
-                return rc;
-            };
-            hipError_t e = hipStreamCreateWithFlags(&ssp->hs, hipStreamNonBlocking);
+            e = hipStreamCreateWithFlags(&ssp->hs, hipStreamNonBlocking);
             if (e != hipSuccess) return fail(hip_error(e, "hipStreamCreate"));
             const size_t G = ssp->G, M = ssp->M;
             for (auto pr : {std::make_pair(&ssp->rec, N * sizeof(VisRec)), std::make_pair(&ssp->sorted, N * sizeof(VisRec)),
@@ -218,7 +300,7 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
                             std::make_pair(&ssp->geom, (size_t)ring * sizeof(SnapGeom)),
                             std::make_pair(&ssp->pmax, pmax_words(ring) * sizeof(unsigned)),
                             std::make_pair(&ssp->win, (size_t)3 * sizeof(int) * ring),
-                            std::make_pair(&ssp->out, un * un * 4 * (size_t)q)}) {
+                            std::make_pair(&ssp->out, un * un * 4)}) {
                 e = pr.first->ensure(pr.second);
                 if (e != hipSuccess) return fail(hip_error(e, "hipMalloc (server)"));
             }
@@ -232,19 +314,6 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
             }
             e = hipMemcpy(ssp->win.p, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice);
             if (e != hipSuccess) return fail(hip_error(e, "hipMemcpy (server)"));
-            for (int s = 0; s < q; ++s) {
-                auto sl = std::make_unique<InSlot>();
-                sl->srv = srv;
-                sl->stream_id = k;
-                const size_t vb = N * esize(ssp->cfg.vis_type);
-                if ((e = sl->vr.ensure(vb)) != hipSuccess || (e = sl->vi.ensure(vb)) != hipSuccess ||
-                    (e = sl->b.ensure(N * 8)) != hipSuccess || (e = sl->v.ensure(36)) != hipSuccess ||
-                    (e = sl->host.ensure(2 * vb + N * 8 + 36)) != hipSuccess ||
-                    (e = sl->result.ensure(un * un * 4)) != hipSuccess ||
-                    (e = hipEventCreateWithFlags(&sl->done, hipEventDisableTiming)) != hipSuccess)
-                    return fail(hip_error(e, "allocation (server slot)"));
-                ssp->slots.push_back(std::move(sl));
-            }
             ssp->latest.assign(un * un, 0.0f);
         }
         *out = srv;
@@ -260,51 +329,70 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     if (!srv || !vis_re || !vis_im || !bin || !vin) return set_error(FIP_ERR_ARG, "null argument");
     if (sid < 0 || sid >= (int)srv->streams.size()) return set_error(FIP_ERR_ARG, "bad stream id");
     SubStream& ss = *srv->streams[sid];
+    const double t_in = now_s();
+    // Admission: this snapshot's reservation must fit the pool's free space now.
     InSlot* sl = nullptr;
-    for (auto& s : ss.slots) {
-        int expect = 0;
-        if (s->busy.compare_exchange_strong(expect, 1)) {
-            sl = s.get();
-            break;
+    {
+        std::lock_guard<std::mutex> lk(srv->amu);
+        size_t off = 0;
+        if (srv->alloc.take(ss.lay.bytes, &off)) {
+            if (srv->idle.empty()) {
+                auto fresh = std::make_unique<InSlot>();
+                fresh->srv = srv;
+                const hipError_t e = hipEventCreateWithFlags(&fresh->done, hipEventDisableTiming);
+                if (e != hipSuccess) {
+                    srv->alloc.give(off, ss.lay.bytes);
+                    return hip_error(e, "hipEventCreate (server)");
+                }
+                srv->idle.push_back(fresh.get());
+                srv->slots.push_back(std::move(fresh));
+            }
+            sl = srv->idle.back();
+            srv->idle.pop_back();
+            sl->off = off;
+            sl->bytes = ss.lay.bytes;
         }
     }
     if (!sl) {
         std::lock_guard<std::mutex> lk(srv->mu);
         srv->refused += 1;
-        return set_error(FIP_ERR_BUDGET, "all input slots of this stream are in flight");
+        return set_error(FIP_ERR_BUDGET, "input pool of the HBM budget is full: snapshot refused");
     }
-    Context* C = nullptr;
-    if (int rc = Context::get(&C)) {
-        sl->busy.store(0);
+    sl->stream_id = sid;
+    bool enqueued = false;   // work reading the reservation may be in flight
+    auto undo = [&](int rc) {
+        if (enqueued) (void)hipStreamSynchronize(ss.hs);
+        std::lock_guard<std::mutex> lk(srv->amu);
+        srv->alloc.give(sl->off, sl->bytes);
+        srv->idle.push_back(sl);
         return rc;
-    }
+    };
+    Context* C = nullptr;
+    if (int rc = Context::get(&C)) return undo(rc);
     std::lock_guard<std::recursive_mutex> lk(C->mu);
     const size_t N = ss.cfg.num_baselines, I = ss.cfg.image_size;
     const size_t vb = N * esize(ss.cfg.vis_type);
     const size_t un = I / ss.cfg.unit_size;
-    char* h = static_cast<char*>(sl->host.p);
+    const InLayout& L = ss.lay;
+    char* h = static_cast<char*>(srv->hpool.p) + sl->off;
+    char* d = srv->pool.as<char>() + sl->off;
     std::memcpy(h, vis_re, vb);
-    std::memcpy(h + vb, vis_im, vb);
-    std::memcpy(h + 2 * vb, bin, N * 8);
-    std::memcpy(h + 2 * vb + N * 8, vin, 36);
-    sl->arrival = arrival_s >= 0.0 ? arrival_s : now_s();
+    std::memcpy(h + L.vi, vis_im, vb);
+    std::memcpy(h + L.b, bin, N * 8);
+    std::memcpy(h + L.v, vin, 36);
+    sl->arrival = arrival_s >= 0.0 ? arrival_s : t_in;
     const long idx = ss.submitted;
     const int slot = (int)(idx % srv->ring);
     sl->window = idx >= 2 ? idx - 2 : -1;
-    auto undo = [&](int rc) {
-        sl->busy.store(0);
-        return rc;
-    };
     hipStream_t st = ss.hs;
 #define SRV_HIP(x)                                              \
     do {                                                        \
         hipError_t e_ = (x);                                    \
         if (e_ != hipSuccess) return undo(hip_error(e_, #x));   \
     } while (0)
-    SRV_HIP(hipMemcpyAsync(sl->vr.p, h, vb, hipMemcpyHostToDevice, st));
-    SRV_HIP(hipMemcpyAsync(sl->vi.p, h + vb, vb, hipMemcpyHostToDevice, st));
-    SRV_HIP(hipMemcpyAsync(sl->b.p, h + 2 * vb, N * 8, hipMemcpyHostToDevice, st));
-    SRV_HIP(hipMemcpyAsync(sl->v.p, h + 2 * vb + N * 8, 36, hipMemcpyHostToDevice, st));
+    // one copy of the contiguous inputs (vis_re .. vin) of the reservation
+    enqueued = true;
+    SRV_HIP(hipMemcpyAsync(d, h, L.res, hipMemcpyHostToDevice, st));
     WorkView w;
     w.rec = ss.rec.as<VisRec>();
     w.bkslot = ss.bkslot.as<int2>();
@@ -319,7 +407,8 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     w.work = ss.work.as<unsigned>();
     w.pmax = ss.pmax.as<unsigned>() + pmax_index(slot, 0, 0);
     w.stride = N;
-    InputView in{sl->vr.p, sl->vi.p, sl->b.as<float>(), sl->v.as<float>(), nullptr, N, (int)N, ss.cfg.vis_type};
+    InputView in{d, d + L.vi, reinterpret_cast<const float*>(d + L.b), reinterpret_cast<const float*>(d + L.v),
+                 nullptr, N, (int)N, ss.cfg.vis_type};
     const PlanTables& p = ss.plan->t;
     const double scale = (double)ss.cfg.cell_size * (double)p.G;
     SRV_HIP(launch_reset(in, w, p, 1, ss.cfg.cell_size, nullptr, (int)N, st));
@@ -336,7 +425,7 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
         wr.img = ss.img.as<float>();
         float* dout = ss.out.as<float>();
         SRV_HIP(launch_trigger(wr, p, ss.win.as<int>() + 3 * slot, 1, (int)ss.cfg.unit_size, dout, st));
-        SRV_HIP(hipMemcpyAsync(sl->result.p, dout, un * un * 4, hipMemcpyDeviceToHost, st));
+        SRV_HIP(hipMemcpyAsync(h + L.res, dout, un * un * 4, hipMemcpyDeviceToHost, st));
     }
     {
         std::lock_guard<std::mutex> lk2(srv->mu);
@@ -427,17 +516,12 @@ void fip_server_destroy(fip_server* srv)
         for (DevBuf* b : {&ss->rec, &ss->bkslot, &ss->sorted, &ss->hist, &ss->bstart, &ss->kt, &ss->pre, &ss->work,
                           &ss->img, &ss->meta, &ss->geom, &ss->pmax, &ss->win, &ss->out})
             b->release();
-        for (auto& sl : ss->slots) {
-            sl->vr.release();
-            sl->vi.release();
-            sl->b.release();
-            sl->v.release();
-            sl->host.release();
-            sl->result.release();
-            if (sl->done) (void)hipEventDestroy(sl->done);
-        }
         if (ss->hs) (void)hipStreamDestroy(ss->hs);
     }
+    for (auto& sl : srv->slots)
+        if (sl->done) (void)hipEventDestroy(sl->done);
+    srv->pool.release();
+    srv->hpool.release();
     delete srv;
 }
 

@@ -269,13 +269,18 @@ typedef struct {
 } fip_stream_config;
 /* Create a server with nstreams sub-array streams, a device ring of `ring`
  * image slots per stream and an HBM budget in bytes for all device state.
- * Returns FIP_ERR_BUDGET if even one snapshot per stream cannot fit. */
+ * The streams' fixed state (work buffers, image ring) plus one snapshot of
+ * the largest stream's inputs must fit (else FIP_ERR_BUDGET); the rest of
+ * the budget, up to `ring` snapshots per stream, is an input pool shared by
+ * all streams, from which each arrival reserves its inputs. */
 int fip_server_create(const fip_stream_config *streams, int nstreams, int ring,
                       size_t hbm_budget_bytes, fip_server **out);
 /* Submit one snapshot (host arrays) for stream `sid` with its arrival time
  * in seconds of the monotonic clock (CLOCK_MONOTONIC, std::steady_clock,
  * Python's time.monotonic()); a negative value stamps the submit time.
- * Returns FIP_ERR_BUDGET if it is refused (no free input slot). */
+ * Admission per arrival: returns FIP_ERR_BUDGET (and counts the snapshot
+ * as refused) when its inputs do not fit the free part of the input pool;
+ * the reservation is released when the snapshot's tLISI is on the host. */
 int fip_server_submit(fip_server *srv, int sid, const void *vis_re,
                       const void *vis_im, const float *bin, const float *vin,
                       double arrival_s);

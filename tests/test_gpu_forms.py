@@ -304,6 +304,55 @@ def test_serving_streams_match_pipe(fip):
         srv.close()
 
 
+def test_serving_admission_per_arrival(fip):
+    """The HBM budget is applied to every arrival: with a budget that leaves
+    room for one snapshot's inputs beyond the stream's fixed state, submits
+    made back to back are refused while an earlier snapshot is in flight,
+    every admitted snapshot completes, and after a drain a submit is admitted
+    again."""
+    cfg = [dict(num_baselines=2016, image_size=256, cell_size=SMALL["cell"], unit_size=16)]
+    # Smallest budget the server accepts (to 256 bytes): FIP_ERR_BUDGET is
+    # decided before anything is allocated.
+    lo, hi = 1 << 16, 1 << 30
+    while hi - lo > 256:
+        mid = (lo + hi) // 2
+        try:
+            fip.Server(cfg, ring=8, hbm_budget_bytes=mid).close()
+            hi = mid
+        except fip.FipError as e:
+            assert e.status == -7
+            lo = mid
+    d = synth_workload(fip, SMALL, nsnap=4, seed=92)
+    srv = fip.Server(cfg, ring=8, hbm_budget_bytes=hi)
+    try:
+        admitted = refused = 0
+        for k in range(40):
+            j = k % 4
+            if srv.submit(0, d["vis_re"][j], d["vis_im"][j], d["bin"][j], d["vin"][j]):
+                admitted += 1
+            else:
+                refused += 1
+        srv.drain()
+        st = srv.stats()
+        print(f"budget {hi} bytes: admitted {admitted}, refused {refused}, processed {st['processed']}")
+        assert refused > 0 and admitted > 0
+        assert st["processed"] == admitted and st["refused"] == refused
+        assert srv.submit(0, d["vis_re"][0], d["vis_im"][0], d["bin"][0], d["vin"][0])
+        srv.drain()
+        assert srv.stats()["processed"] == admitted + 1
+    finally:
+        srv.close()
+    # with room for every snapshot of the ring nothing is refused
+    srv = fip.Server(cfg, ring=8, hbm_budget_bytes=1 << 30)
+    try:
+        for k in range(8):
+            assert srv.submit(0, d["vis_re"][k % 4], d["vis_im"][k % 4], d["bin"][k % 4], d["vin"][k % 4])
+        srv.drain()
+        assert srv.stats()["refused"] == 0
+    finally:
+        srv.close()
+
+
 def test_serving_budget_refuses(fip):
     with pytest.raises(fip.FipError) as ei:
         fip.Server([dict(num_baselines=130816, image_size=4096, cell_size=7.27e-6, unit_size=32)],
