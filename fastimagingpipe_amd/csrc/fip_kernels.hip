@@ -875,9 +875,24 @@ __device__ __forceinline__ GeomF geom_f(const SnapGeom& g)
 // power of two, rounded half up to integers and summed as int32; the scale
 // keeps every pixel sum below 2^29 (plus half a unit per contribution), so
 // each sum is exact and independent of the order of the LDS atomics.
+// floor(x) and floor(x + 0.5) as int32 in one instruction each (the compiler
+// keeps floor and the conversion separate without finite-math flags).
+__device__ __forceinline__ int cvt_floor_i32(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ int cvt_round_half_up_i32(float x)
+{
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 __device__ __forceinline__ void acc_exact(int* a, float x)
 {
-    atomicAdd(a, (int)(x + 0.5f));
+    atomicAdd(a, cvt_round_half_up_i32(x));
 }
 
 // Per target tile: bounding box and per-row ranges of the source pixels whose
@@ -1072,88 +1087,116 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
                 for (int k = (a - f0 + 63) >> 6; (k << 6) < b - f0; ++k) crow[k] = (unsigned char)r;
             }
             __syncthreads();
-            for (int base = 0; base < nf; base += NT) {
-                const int fl = base + tid;
-                if (fl >= nf) break;
-                const int f = f0 + fl;
-                int r = crow[fl >> 6];
-                while (rofs[r + 1] <= f) ++r;
-                const int gi = rlo[r] + (f - rofs[r]);
-                const int jj = j0 + rb + r;
-                // Aligned 16-byte loads (I % GN == 0, so a group never crosses
-                // the end of a row).
-                float pv[GN];
-                const float4* src = reinterpret_cast<const float4*>(ps + (size_t)jj * I + GN * gi);
+            // The source loop, specialised on the tile: EDGE for tiles on the
+            // left or top image edge (an integer position on the bound is
+            // outside there), UNB for maps without a curvature bound
+            // (positions checked for size).  Interior tiles of bounded maps,
+            // nearly all of them, take neither check.
+            auto sources = [&](auto EDGE, auto UNB) {
+                constexpr bool edge = decltype(EDGE)::value, unb = decltype(UNB)::value;
+                for (int base = 0; base < nf; base += NT) {
+                    const int fl = base + tid;
+                    if (fl >= nf) break;
+                    const int f = f0 + fl;
+                    int r = crow[fl >> 6];
+                    while (rofs[r + 1] <= f) ++r;
+                    const int gi = rlo[r] + (f - rofs[r]);
+                    const int jj = j0 + rb + r;
+                    // Aligned 16-byte loads (I % GN == 0, so a group never
+                    // crosses the end of a row).
+                    float pv[GN];
+                    const float4* src = reinterpret_cast<const float4*>(ps + (size_t)jj * I + GN * gi);
 #pragma unroll
-                for (int u = 0; u < GN / 4; ++u) {
-                    const float4 q = src[u];
-                    pv[4 * u] = q.x;
-                    pv[4 * u + 1] = q.y;
-                    pv[4 * u + 2] = q.z;
-                    pv[4 * u + 3] = q.w;
-                }
-                // Group origin in FP64: the affine position of its first pixel
-                // (clamped: positions beyond 1e6 px are rejected below), split
-                // into an integer part O and a fraction.  The pixels are placed
-                // in FP32 relative to O (local coordinates stay within a few
-                // pixels, ~1e-7 px), and only integer parts are recombined.
-                const double a0 = (double)(GN * gi - h), b = (double)(jj - h);
-                const double q1d = fmin(fmax(fma(g.m00, a0, g.m01 * b), -1e6), 1e6);
-                const double q2d = fmin(fmax(fma(g.m10, a0, g.m11 * b), -1e6), 1e6);
-                const double O1 = floor(q1d), O2 = floor(q2d);
-                const float r1 = (float)(q1d - O1), r2 = (float)(q2d - O2);
-                const int c1 = (int)O1 + h - C0, c2 = (int)O2 + h - R0;   // target offsets in the tile
-                // SIN direction cosines along the group: l = -cell (P1 - c0),
-                // m = cell (P2 - c0), c0 = I/2 + 1 (FIPkernels.cu:315-316); the
-                // origin minus one is an exact FP32 integer.
-                const float l0 = -gf.delta * ((float)(O1 - 1.0) + r1), dl = -gf.delta * gf.m00;
-                const float n0 = gf.delta * ((float)(O2 - 1.0) + r2), dn = gf.delta * gf.m10;
+                    for (int u = 0; u < GN / 4; ++u) {
+                        const float4 q = src[u];
+                        pv[4 * u] = q.x;
+                        pv[4 * u + 1] = q.y;
+                        pv[4 * u + 2] = q.z;
+                        pv[4 * u + 3] = q.w;
+                    }
+                    // Group origin in FP64: the affine position of its first
+                    // pixel (clamped: positions beyond 1e6 px are rejected
+                    // below), split into an integer part O and a fraction.  The
+                    // pixels are placed in FP32 relative to O (local
+                    // coordinates stay within a few pixels, ~1e-7 px), and only
+                    // integer parts are recombined.
+                    const double a0 = (double)(GN * gi - h), b = (double)(jj - h);
+                    const double q1d = fmin(fmax(fma(g.m00, a0, g.m01 * b), -1e6), 1e6);
+                    const double q2d = fmin(fmax(fma(g.m10, a0, g.m11 * b), -1e6), 1e6);
+                    const double O1 = floor(q1d), O2 = floor(q2d);
+                    const float r1 = (float)(q1d - O1), r2 = (float)(q2d - O2);
+                    // target offsets of the origin in the tile, relative to the
+                    // accepted ranges' starts, and the LDS byte address of the
+                    // origin's apron cell
+                    const int c1 = (int)O1 + h - C0, c2 = (int)O2 + h - R0;
+                    const int e1 = c1 - cxlo, e2 = c2 - rylo;
+                    const unsigned gb = 4u * (unsigned)((c2 + 1) * A + c1 + 1);
+                    // SIN direction cosines along the group: l = -cell (P1 - c0),
+                    // m = cell (P2 - c0), c0 = I/2 + 1 (FIPkernels.cu:315-316);
+                    // the origin minus one is an exact FP32 integer.
+                    const float l0 = -gf.delta * ((float)(O1 - 1.0) + r1), dl = -gf.delta * gf.m00;
+                    const float n0 = gf.delta * ((float)(O2 - 1.0) + r2), dn = gf.delta * gf.m10;
 #pragma unroll
-                for (int k = 0; k < GN; ++k) {
-                    float o1 = fmaf((float)k, gf.m00, r1), o2 = fmaf((float)k, gf.m10, r2);
-                    if (gf.sin_on) {
-                        const float l = fmaf((float)k, dl, l0), mm = fmaf((float)k, dn, n0);
-                        const float rr = l * l + mm * mm;
-                        float np = 0.0f;
-                        if (gf.series) {
-                            // every source of the image has r^2 <= 0.01 (checked in k_reset)
-                            np = rr * fmaf(rr, fmaf(rr, fmaf(rr, 0.0390625f, 0.0625f), 0.125f), 0.5f);
-                        } else if (rr <= 1.0f) {
-                            // 1 - sqrt(1 - r^2) = r^2 / (1 + sqrt(1 - r^2)); hardware sqrt/rcp
-                            np = rr * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - rr));
+                    for (int k = 0; k < GN; ++k) {
+                        float o1 = fmaf((float)k, gf.m00, r1), o2 = fmaf((float)k, gf.m10, r2);
+                        if (gf.sin_on) {
+                            const float l = fmaf((float)k, dl, l0), mm = fmaf((float)k, dn, n0);
+                            const float rr = l * l + mm * mm;
+                            float np = 0.0f;
+                            if (gf.series) {
+                                // every source of the image has r^2 <= 0.01 (checked in k_reset)
+                                np = rr * fmaf(rr, fmaf(rr, fmaf(rr, 0.0390625f, 0.0625f), 0.125f), 0.5f);
+                            } else if (rr <= 1.0f) {
+                                // 1 - sqrt(1 - r^2) = r^2 / (1 + sqrt(1 - r^2)); hardware sqrt/rcp
+                                np = rr * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - rr));
+                            }
+                            o1 = fmaf(gf.sx, np, o1);
+                            o2 = fmaf(gf.sy, np, o2);
                         }
-                        o1 = fmaf(gf.sx, np, o1);
-                        o2 = fmaf(gf.sy, np, o2);
-                    }
-                    bool fin = true;
-                    if (unbounded) {
-                        // finite and small enough for the integer recombination
-                        fin = fabsf(o1) < 1048576.0f && fabsf(o2) < 1048576.0f;
-                        o1 = fin ? o1 : 0.0f;
-                        o2 = fin ? o2 : 0.0f;
-                    }
-                    const float f1 = floorf(o1), f2 = floorf(o2);
-                    const float wl1 = o1 - f1, wm1 = o2 - f2;
-                    const int ca = c1 + (int)f1, ra = c2 + (int)f2;
-                    // Tile apron and |LL|, |MM| < h - 1 (FIPkernels.cu:389) as one
-                    // unsigned range per axis; on the left / top image edge an
-                    // integer position on the bound is outside.
-                    bool ok = fin && (unsigned)(ca - cxlo) <= (unsigned)cxn && (unsigned)(ra - rylo) <= (unsigned)ryn;
-                    if (cedge || redge)
-                        ok = ok && !(cedge && ca == cxlo && wl1 == 0.0f) && !(redge && ra == rylo && wm1 == 0.0f);
-                    if (ok) {
-                        const float vs = pv[k] * vscale;
-                        const float t0 = (1.0f - wl1) * vs, t1 = wl1 * vs, om = 1.0f - wm1;
-                        // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL); a
-                        // zero weight on an integer position adds an exact zero.
-                        int* p0 = acc + mad_u24((unsigned)(ra + 1), (unsigned)A, (unsigned)(ca + 1));
-                        acc_exact(p0, t0 * om);
-                        acc_exact(p0 + A, t0 * wm1);
-                        acc_exact(p0 + 1, t1 * om);
-                        acc_exact(p0 + A + 1, t1 * wm1);
+                        bool fin = true;
+                        if constexpr (unb) {
+                            // finite and small enough for the integer recombination
+                            fin = fabsf(o1) < 1048576.0f && fabsf(o2) < 1048576.0f;
+                            o1 = fin ? o1 : 0.0f;
+                            o2 = fin ? o2 : 0.0f;
+                        }
+                        // floor as an integer and the fraction o - floor(o) in
+                        // [0, 1): the bilinear weights of the two axes
+                        const int f1 = cvt_floor_i32(o1), f2 = cvt_floor_i32(o2);
+                        const float wl1 = __builtin_amdgcn_fractf(o1), wm1 = __builtin_amdgcn_fractf(o2);
+                        // Tile apron and |LL|, |MM| < h - 1 (FIPkernels.cu:389) as
+                        // one unsigned range per axis; on the left / top image
+                        // edge an integer position on the bound is outside.
+                        const int u1 = f1 + e1, u2 = f2 + e2;
+                        bool ok = (unsigned)u1 <= (unsigned)cxn && (unsigned)u2 <= (unsigned)ryn;
+                        if constexpr (unb) ok = ok && fin;
+                        if constexpr (edge)
+                            ok = ok && !(cedge && u1 == 0 && wl1 == 0.0f) && !(redge && u2 == 0 && wm1 == 0.0f);
+                        if (ok) {
+                            // FIPkernels.cu:395-410: (fM,fL), (cM,fL), (fM,cL), (cM,cL)
+                            // with weights (1-a)(1-b), (1-a)b, a(1-b), ab; each
+                            // product is split off its row by one subtraction.  A
+                            // zero weight on an integer position adds an exact zero.
+                            const float vs = pv[k] * vscale;
+                            const float t1 = wl1 * vs, t0 = vs - t1;
+                            const float p01 = t0 * wm1, p11 = t1 * wm1;
+                            const float p00 = t0 - p01, p10 = t1 - p11;
+                            int* p0 = reinterpret_cast<int*>(reinterpret_cast<char*>(acc) +
+                                                             (gb + (unsigned)(__mul24(f2, 4 * A) + 4 * f1)));
+                            acc_exact(p0, p00);
+                            acc_exact(p0 + A, p01);
+                            acc_exact(p0 + 1, p10);
+                            acc_exact(p0 + A + 1, p11);
+                        }
                     }
                 }
-            }
+            };
+            if (unbounded)
+                sources(std::true_type{}, std::true_type{});
+            else if (cedge || redge)
+                sources(std::true_type{}, std::false_type{});
+            else
+                sources(std::false_type{}, std::false_type{});
             __syncthreads();   // crow is rewritten by the next window
         }
         __syncthreads();   // rlo/rcnt are rewritten by the next row batch
