@@ -437,21 +437,53 @@ __device__ __forceinline__ int tap_row(const ColRec& c, int t1, int M)
     return c.mirror ? ((t1 == -M) ? -M : -t1) : t1;
 }
 
+// floor(x) and floor(x + 0.5) as int32 in one instruction each (the compiler
+// keeps floor and the conversion separate without finite-math flags).
+__device__ __forceinline__ int cvt_floor_i32(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ int cvt_round_half_up_i32(float x)
+{
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// ES inside its support, |x| <= 1 (as es_f without the range test).
+__device__ __forceinline__ float es_in(float x)
+{
+    const float xx = x * x;
+    const float s = __builtin_amdgcn_sqrtf(1.0f - xx);
+    const float y = -(float)(kBeta * 1.4426950408889634) * xx * __builtin_amdgcn_rcpf(1.0f + s);
+    return __builtin_amdgcn_exp2f(y);
+}
+
 // Row taps of one prepared record, accumulated as integers into the LDS
 // column (re, im interleaved; row r at padded index of r mod G), each scaled
-// by its block's power of two.
+// by its block's power of two.  A mirrored record adds at row -t1 (mod G,
+// which also maps the edge tap t1 = -g/2 onto itself).  The tap offsets
+// (t1 - p1)/4 step by 1/4 from the first tap; each contribution is rounded
+// half up to an integer.  rbs is a power of two.
 __device__ __forceinline__ void grid_taps(const ColRec& c, int M, int G, int rbs, const float* sc, int* col)
 {
     const int lo1 = max(c.b1 - 4 + (c.f1 > 0.0f ? 1 : 0), -M);
     const int hi1 = min(c.b1 + 4, M - 1);
-    for (int t1 = lo1; t1 <= hi1; ++t1) {
-        const int row = tap_row(c, t1, M);
+    float x = ((float)(lo1 - c.b1) - c.f1) * 0.25f;
+    const int sgn = c.mirror ? -1 : 1;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int t1 = lo1 + j;
+        if (t1 > hi1) break;
+        const int row = sgn * t1;
         const int idx = row < 0 ? row + G : row;
         const int q = 2 * lds_pad(idx);
-        const float w1 = es_f(((float)(t1 - c.b1) - c.f1) * 0.25f);
-        const float k = sc[idx / rbs];
-        atomicAdd(&col[q], (int)rintf((c.vr * w1) * k));
-        atomicAdd(&col[q + 1], (int)rintf((c.vi * w1) * k));
+        const float wk = es_in(x) * sc[(unsigned)idx / (unsigned)rbs];
+        atomicAdd(&col[q], cvt_round_half_up_i32(c.vr * wk));
+        atomicAdd(&col[q + 1], cvt_round_half_up_i32(c.vi * wk));
+        x += 0.25f;
     }
 }
 
@@ -875,21 +907,6 @@ __device__ __forceinline__ GeomF geom_f(const SnapGeom& g)
 // power of two, rounded half up to integers and summed as int32; the scale
 // keeps every pixel sum below 2^29 (plus half a unit per contribution), so
 // each sum is exact and independent of the order of the LDS atomics.
-// floor(x) and floor(x + 0.5) as int32 in one instruction each (the compiler
-// keeps floor and the conversion separate without finite-math flags).
-__device__ __forceinline__ int cvt_floor_i32(float x)
-{
-    int r;
-    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-    return r;
-}
-__device__ __forceinline__ int cvt_round_half_up_i32(float x)
-{
-    int r;
-    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-    return r;
-}
-
 __device__ __forceinline__ void acc_exact(int* a, float x)
 {
     atomicAdd(a, cvt_round_half_up_i32(x));
