@@ -186,6 +186,7 @@ int Context::get(Context** out)
         FIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         FIP_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         for (hipEvent_t& e : c->in_ready) FIP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->host_copy = new CopyPool(3);
         FIP_HIP(hipEventCreateWithFlags(&c->work_done_, hipEventDisableTiming));
         for (auto& pair : c->ev_) {
             FIP_HIP(hipEventCreate(&pair[0]));
@@ -250,7 +251,10 @@ int Context::plan(size_t image, Plan** out)
 Tuning default_tuning(size_t image)
 {
     Tuning t;
-    t.stage1_records = 4;         // gridding records per thread in registers
+    // Gridding records per thread in registers: 2, the autotuner's choice at
+    // 4096^2 and 2048^2 on MI355X (each thread's records stay resident up
+    // to 1024 per column; heavier columns take further passes).
+    t.stage1_records = 2;
     // Rows per C2R workgroup: four rows of up to 768 points, two of 1536 and
     // 3072 (each pair of rows read with one 16-byte load per K entry: 6%
     // faster than four rows at 2048), one of 6144 (two rows of it hold
