@@ -36,6 +36,7 @@ SOURCES = [
     "fip_noalloc.hip",
     "fip_plan.cpp",
     "fip_context.cpp",
+    "fip_copy.cpp",
     "fip_api.cpp",
     "fip_synth.cpp",
     "fip_fits.cpp",
