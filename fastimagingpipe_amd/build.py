@@ -36,7 +36,6 @@ SOURCES = [
     "fip_noalloc.hip",
     "fip_plan.cpp",
     "fip_context.cpp",
-    "fip_copy.cpp",
     "fip_api.cpp",
     "fip_synth.cpp",
     "fip_fits.cpp",

@@ -99,7 +99,7 @@ int pipe_host(const void* vis_re, const void* vis_im, int vt, const float* bin, 
             char* h = hs + (size_t)s * region;
             char* d = ds + (size_t)s * region;
             for (const auto& pt : parts) {
-                C->host_copy->copy(h + pt.off, pt.src, pt.bytes);
+                std::memcpy(h + pt.off, pt.src, pt.bytes);
                 FIP_HIP(hipMemcpyAsync(d + pt.off, h + pt.off, pt.bytes, hipMemcpyHostToDevice, cs));
             }
             FIP_HIP(hipEventRecord(C->in_ready[s], cs));

@@ -1,19 +1,15 @@
 // Device context: stream, per-size plan tables, work buffers and timing.
 #pragma once
 
-#include "fip_copy.hpp"
 #include "fip_internal.hpp"
 
 #include <hip/hip_runtime.h>
 
-#include <atomic>
-#include <condition_variable>
 #include <cstddef>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
-#include <thread>
 #include <vector>
 
 namespace fip {
@@ -105,7 +101,6 @@ public:
     // copy_stream; in_ready[s] marks snapshot s's region as copied.
     DevBuf in_snap;
     hipStream_t copy_stream = nullptr;
-    CopyPool* host_copy = nullptr;   // created with the context, never destroyed
     hipEvent_t in_ready[3] = {};
     // Preprocessing (projection, collapse and the raw-input chain): device
     // inputs of the host-array entry points, per-block moment partials,

@@ -376,9 +376,9 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     const InLayout& L = ss.lay;
     char* h = static_cast<char*>(srv->hpool.p) + sl->off;
     char* d = srv->pool.as<char>() + sl->off;
-    C->host_copy->copy(h, vis_re, vb);
-    C->host_copy->copy(h + L.vi, vis_im, vb);
-    C->host_copy->copy(h + L.b, bin, N * 8);
+    std::memcpy(h, vis_re, vb);
+    std::memcpy(h + L.vi, vis_im, vb);
+    std::memcpy(h + L.b, bin, N * 8);
     std::memcpy(h + L.v, vin, 36);
     sl->arrival = arrival_s >= 0.0 ? arrival_s : t_in;
     const long idx = ss.submitted;
