@@ -186,7 +186,6 @@ int Context::get(Context** out)
         FIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         FIP_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         for (hipEvent_t& e : c->in_ready) FIP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        c->host_copy = new CopyPool(3);
         FIP_HIP(hipEventCreateWithFlags(&c->work_done_, hipEventDisableTiming));
         for (auto& pair : c->ev_) {
             FIP_HIP(hipEventCreate(&pair[0]));

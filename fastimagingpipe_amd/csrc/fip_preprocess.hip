@@ -235,6 +235,10 @@ extern "C" int fip_project(const double* uvw, size_t nrows, const double* freqs,
         if (int rc = Context::get(&C)) return rc;
         std::lock_guard<std::recursive_mutex> lk(C->mu);
         hipStream_t st = C->stream;
+        // The pinned and device preprocessing buffers are shared with
+        // fip_pipe_raw_device, whose last copies may still be in flight on a
+        // caller's stream: wait for them first.
+        if (int rc = C->order_work(st)) return rc;
         FIP_HIP(hipStreamSynchronize(st));
         FIP_HIP(C->pp_uvw.ensure(nrows * 3 * sizeof(double)));
         FIP_HIP(C->pp_freq.ensure(nchan * sizeof(double)));
@@ -246,6 +250,7 @@ extern "C" int fip_project(const double* uvw, size_t nrows, const double* freqs,
                                     C->pp_in.as<float2>(), ns, nullptr, V, st))
             return rc;
         FIP_HIP(hipMemcpyAsync(bin, C->pp_in.p, ns * sizeof(float2), hipMemcpyDeviceToHost, st));
+        if (int rc = C->mark_work(st)) return rc;
         FIP_HIP(hipStreamSynchronize(st));
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) vin[3 * a + b] = (float)V[0][a][b];
@@ -267,6 +272,7 @@ extern "C" int fip_collapse(const float* xx_re, const float* xx_im, const float*
         if (int rc = Context::get(&C)) return rc;
         std::lock_guard<std::recursive_mutex> lk(C->mu);
         hipStream_t st = C->stream;
+        if (int rc = C->order_work(st)) return rc;
         FIP_HIP(hipStreamSynchronize(st));
         const size_t fb = n * sizeof(float);
         FIP_HIP(C->pp_in.ensure(fb * 8 + 2 * n + 64));
