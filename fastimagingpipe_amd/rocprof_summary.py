@@ -1,9 +1,11 @@
 """Summaries of rocprofv3 CSV output for this library's kernels.
 
-    python -m fastimagingpipe_amd.rocprof_summary stats DIR
+    python -m fastimagingpipe_amd.rocprof_summary stats DIR [--last N]
         Per-kernel call count and average / total duration from the
         ``*kernel_stats.csv`` that ``rocprofv3 --kernel-trace --stats -f csv``
-        writes under DIR.
+        writes under DIR.  With --last N, from the ``*kernel_trace.csv``
+        instead, over each kernel's last N dispatches only (the timed steps of
+        a run, without its warm-up), with their median and spread.
 
     python -m fastimagingpipe_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR \
             --copy-bytes N [--json OUT]
@@ -154,11 +156,37 @@ def traffic(fetch_dir: str, write_dir: str, copy_bytes: int) -> dict:
     return out
 
 
+def kernel_trace_last(directory: str, last: int) -> list[dict]:
+    """Per kernel name: the last `last` dispatches of the kernel trace (in
+    dispatch order), their mean, median, min and max duration."""
+    per = defaultdict(list)
+    for path in _find(directory, "kernel_trace.csv"):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                name = _col(r, "Kernel_Name", "KernelName", "Name")
+                start, end = int(_col(r, "Start_Timestamp")), int(_col(r, "End_Timestamp"))
+                per[name].append((int(_col(r, "Dispatch_Id", "Correlation_Id")), end - start))
+    rows = []
+    for name, v in per.items():
+        v.sort()
+        d = sorted(x for _, x in v[-last:])
+        n = len(d)
+        rows.append({"name": name, "calls": n, "total_ns": float(sum(d)), "avg_ns": sum(d) / n,
+                     "median_ns": float(d[n // 2] if n % 2 else 0.5 * (d[n // 2 - 1] + d[n // 2])),
+                     "min_ns": float(d[0]), "max_ns": float(d[-1])})
+    tot = sum(r["total_ns"] for r in rows) or 1.0
+    for r in rows:
+        r["pct"] = 100.0 * r["total_ns"] / tot
+    rows.sort(key=lambda r: -r["total_ns"])
+    return rows
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("stats")
     a.add_argument("dir")
+    a.add_argument("--last", type=int, default=0)
     b = sub.add_parser("traffic")
     b.add_argument("fetch_dir")
     b.add_argument("write_dir")
@@ -180,6 +208,15 @@ def main(argv=None) -> int:
             if all(num in tab[st] and den in tab[st] for st in order):
                 print(f"{label:<32}" + "".join(
                     f"{(tab[st][num] / tab[st][den] if tab[st][den] else float('nan')):>14.3f}" for st in order))
+        return 0
+    if args.cmd == "stats" and args.last > 0:
+        rows = kernel_trace_last(args.dir, args.last)
+        print(f"{'kernel':<72} {'stage':<12} {'calls':>6} {'avg_us':>10} {'median_us':>10} {'min_us':>8} "
+              f"{'max_us':>8} {'pct':>6}")
+        for r in rows:
+            print(f"{short(r['name']):<72} {stage_of(r['name']) or '-':<12} {r['calls']:>6} "
+                  f"{r['avg_ns'] / 1e3:>10.2f} {r['median_ns'] / 1e3:>10.2f} {r['min_ns'] / 1e3:>8.2f} "
+                  f"{r['max_ns'] / 1e3:>8.2f} {r['pct']:>6.2f}")
         return 0
     if args.cmd == "stats":
         rows = kernel_stats(args.dir)

@@ -73,3 +73,26 @@ def test_kernel_stats_are_sorted_by_total_time(tmp_path):
     assert [r["name"] for r in rows] == [COPY, SPLAT, C2R]
     assert R.stage_of(rows[1]["name"]) == "splat" and R.stage_of(rows[0]["name"]) is None
     assert rows[2]["avg_ns"] == pytest.approx(1.24e5) and rows[2]["calls"] == 25
+
+
+def test_trace_statistics_over_the_last_dispatches(tmp_path):
+    """--last N averages each kernel's last N dispatches (dispatch order), so
+    a warm-up dispatch before them does not count."""
+    path = tmp_path / "trace"
+    os.makedirs(path)
+    with open(path / "run_kernel_trace.csv", "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kind", "Dispatch_Id", "Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        t = 0
+        durs = {SPLAT: [200000, 110000, 120000, 100000], C2R: [90000, 100000]}
+        for i in range(4):
+            for k, d in durs.items():
+                if i < len(d):
+                    w.writerow(["KERNEL_DISPATCH", 10 * i + (k == C2R), k, t, t + d[i]])
+                    t += d[i] + 1000
+    rows = {r["name"]: r for r in R.kernel_trace_last(str(path), 3)}
+    assert rows[SPLAT]["calls"] == 3
+    assert rows[SPLAT]["avg_ns"] == pytest.approx(110000.0)
+    assert rows[SPLAT]["median_ns"] == pytest.approx(110000.0)
+    assert rows[SPLAT]["min_ns"] == 100000.0 and rows[SPLAT]["max_ns"] == 120000.0
+    assert rows[C2R]["calls"] == 2 and rows[C2R]["avg_ns"] == pytest.approx(95000.0)
