@@ -153,3 +153,29 @@ def test_cxx_caller_links_fipipe_by_its_reference_prototype():
     assert "_Z6FIpipePfS_S_S_S_mmmfm" in syms
     deps = subprocess.run(["ldd", caller], capture_output=True, text=True, check=True).stdout
     assert "libfip.so" in deps and "not found" not in deps.split("libfip.so")[1].splitlines()[0]
+
+
+def test_noalloc_argument_errors_before_any_device_use(F):
+    """fip_pipe_device_noalloc refuses what its static array cannot hold before
+    it touches a device or a pointer (the pointers here are never read)."""
+    fake = 256
+    f = F.lib.fip_pipe_device_noalloc
+    assert f(None, fake, fake, fake, fake, 2016, 256, 3, 5.8e-5, 16, None) == -1
+    assert f(fake, fake, fake, fake, fake, 2016, 256, 2, 5.8e-5, 16, None) == -2
+    assert f(fake, fake, fake, fake, fake, 2016, 2048, 3, 5.8e-5, 32, None) == -3   # beyond 1024^2
+    assert b"64 MiB" in F.lib.fip_last_error()
+    assert f(fake, fake, fake, fake, fake, 200000, 1024, 3, 2.9e-5, 32, None) == -3   # work > 64 MiB
+    assert b"bytes of scratch" in F.lib.fip_last_error()
+
+
+def test_raw_chain_argument_errors_before_any_device_use(F):
+    """fip_pipe_raw_device validates its view and shape first (fake, never-read
+    device pointers)."""
+    fake = 256
+    keys = ("uvw", "freqs", "xx_re", "xx_im", "yy_re", "yy_im", "flag0", "flag3", "w0", "w3")
+    ptrs = {k: fake for k in keys}
+    _expect(F, -2, F.pipe_raw_device, ptrs, 2016, 1, 2, 256, 5.8e-5, 16, fake)     # NSNAP
+    _expect(F, -1, F.pipe_raw_device, ptrs, 0, 1, 3, 256, 5.8e-5, 16, fake)        # no rows
+    _expect(F, -3, F.pipe_raw_device, ptrs, 2016, 1, 3, 100, 5.8e-5, 16, fake)     # image size
+    _expect(F, -3, F.pipe_raw_device, ptrs, 1 << 30, 1 << 20, 3, 256, 5.8e-5, 16, fake)   # nrows*nchan
+    _expect(F, -1, F.pipe_raw_device, dict(ptrs, w3=0), 2016, 1, 3, 256, 5.8e-5, 16, fake)
