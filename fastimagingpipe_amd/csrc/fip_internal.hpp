@@ -61,6 +61,7 @@ struct SnapMeta {
 struct SnapGeom {
     double m00, m01, m10, m11;   // affine: (a,b) = source - h  ->  (q1,q2) = target - h
     double i00, i01, i10, i11;   // inverse of the affine part
+    double im00, im10;           // 1/m00, 1/m10 (0 where |m| <= 1e-12)
     double sx, sy;               // SIN shift per unit n' (pixels): -xi/cell, eta/cell
     double delta;                // cell (rad)
     double curv;                 // bound on the SIN map's curvature (1/px), <0: unbounded

@@ -85,11 +85,21 @@ __device__ __forceinline__ int wave_max_i(int v)
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
+// Max over the 64 lanes (all active), returned in every lane: DPP row shifts
+// leave each row of 16 lanes' max in its last lane, and the four row maxima
+// are combined as scalars (max is exact in any order).  10 instructions
+// instead of 30 for the shuffle butterfly.
 __device__ __forceinline__ unsigned wave_max_u(unsigned v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o, 64));
-    return v;
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));   // row_shr:1
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));   // row_shr:2
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));   // row_shr:4
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));   // row_shr:8
+    const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 15);
+    const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)v, 31);
+    const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 47);
+    const unsigned d = (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+    return max(max(a, b), max(c, d));
 }
 __device__ __forceinline__ int wave_sum_i(int v)
 {
@@ -180,6 +190,8 @@ __global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ n
         gm.i01 = -gm.m01 / det;
         gm.i10 = -gm.m10 / det;
         gm.i11 = gm.m00 / det;
+        gm.im00 = fabs(gm.m00) > 1e-12 ? 1.0 / gm.m00 : 0.0;
+        gm.im10 = fabs(gm.m10) > 1e-12 ? 1.0 / gm.m10 : 0.0;
         const double xi = v20 / v22, eta = v21 / v22;
         const double delta = (double)cell;
         gm.sx = -xi / delta;
@@ -884,6 +896,15 @@ __device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c)
     return r;
 }
 
+// a * b + c for |a|, |b| < 2^23 (signed 24-bit) in one full-rate instruction;
+// b is uniform (a scalar register).
+__device__ __forceinline__ int mad_i24(int a, int b, int c)
+{
+    int r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
 // FP32 copy of the geometry the splat needs per source.
 struct GeomF {
     float m00, m10;   // affine step per source column
@@ -959,11 +980,17 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
         if (g.curv >= 0.0) {
             const double t1 = (double)C0 + 0.5 * (TS - 1) - h, t2 = (double)R0 + 0.5 * (TS - 1) - h;
             double q1 = t1, q2 = t2;
+            // At most four fixed-point steps; the residual below bounds what is
+            // left, so stopping once a step moves less than 1e-4 px keeps the
+            // box conservative.
             for (int it = 0; it < 4; ++it) {
                 double o1, o2;
                 sin_shift(g, q1, q2, o1, o2);
-                q1 = t1 - (o1 - q1);
-                q2 = t2 - (o2 - q2);
+                const double n1 = t1 - (o1 - q1), n2 = t2 - (o2 - q2);
+                const bool done = fabs(n1 - q1) + fabs(n2 - q2) < 1e-4;
+                q1 = n1;
+                q2 = n2;
+                if (done) break;
             }
             double o1, o2;
             sin_shift(g, q1, q2, o1, o2);
@@ -1000,8 +1027,8 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
         qb[1] = qh1;
         qb[2] = ql2;
         qb[3] = qh2;
-        qb[4] = fabs(g.m00) > 1e-12 ? 1.0 / g.m00 : 0.0;
-        qb[5] = fabs(g.m10) > 1e-12 ? 1.0 / g.m10 : 0.0;
+        qb[4] = g.im00;
+        qb[5] = g.im10;
         bb[0] = i0;
         bb[1] = i1;
         bb[2] = j0;
@@ -1109,8 +1136,12 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
             // outside there), UNB for maps without a curvature bound
             // (positions checked for size).  Interior tiles of bounded maps,
             // nearly all of them, take neither check.
-            auto sources = [&](auto EDGE, auto UNB) {
+            auto sources = [&](auto EDGE, auto UNB, auto SER) {
                 constexpr bool edge = decltype(EDGE)::value, unb = decltype(UNB)::value;
+                // SIN shift: 1 by its series, 0 by its closed form (a map
+                // without one has sx = sy = 0, which adds exact zeros), -1 as
+                // the geometry's flags say
+                constexpr int ser = decltype(SER)::value;
                 for (int base = 0; base < nf; base += NT) {
                     const int fl = base + tid;
                     if (fl >= nf) break;
@@ -1138,8 +1169,12 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
                     // coordinates stay within a few pixels, ~1e-7 px), and only
                     // integer parts are recombined.
                     const double a0 = (double)(GN * gi - h), b = (double)(jj - h);
-                    const double q1d = fmin(fmax(fma(g.m00, a0, g.m01 * b), -1e6), 1e6);
-                    const double q2d = fmin(fmax(fma(g.m10, a0, g.m11 * b), -1e6), 1e6);
+                    double q1d = fma(g.m00, a0, g.m01 * b), q2d = fma(g.m10, a0, g.m11 * b);
+                    if constexpr (unb) {
+                        // bounded maps keep positions within the image's reach
+                        q1d = fmin(fmax(q1d, -1e6), 1e6);
+                        q2d = fmin(fmax(q2d, -1e6), 1e6);
+                    }
                     const double O1 = floor(q1d), O2 = floor(q2d);
                     const float r1 = (float)(q1d - O1), r2 = (float)(q2d - O2);
                     // target offsets of the origin in the tile, relative to the
@@ -1147,20 +1182,21 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
                     // origin's apron cell
                     const int c1 = (int)O1 + h - C0, c2 = (int)O2 + h - R0;
                     const int e1 = c1 - cxlo, e2 = c2 - rylo;
-                    const unsigned gb = 4u * (unsigned)((c2 + 1) * A + c1 + 1);
+                    unsigned gb = 4u * (unsigned)(__mul24(c2 + 1, A) + c1 + 1);
+                    asm volatile("" : "+v"(gb));   // kept whole, so gb + 4 f1 is one instruction
                     // SIN direction cosines along the group: l = -cell (P1 - c0),
-                    // m = cell (P2 - c0), c0 = I/2 + 1 (FIPkernels.cu:315-316);
-                    // the origin minus one is an exact FP32 integer.
-                    const float l0 = -gf.delta * ((float)(O1 - 1.0) + r1), dl = -gf.delta * gf.m00;
-                    const float n0 = gf.delta * ((float)(O2 - 1.0) + r2), dn = gf.delta * gf.m10;
+                    // m = cell (P2 - c0), c0 = I/2 + 1 (FIPkernels.cu:315-316),
+                    // rounded once from FP64.
+                    const float l0 = (float)fma(-g.delta, q1d, g.delta), dl = -gf.delta * gf.m00;
+                    const float n0 = (float)fma(g.delta, q2d, -g.delta), dn = gf.delta * gf.m10;
 #pragma unroll
                     for (int k = 0; k < GN; ++k) {
                         float o1 = fmaf((float)k, gf.m00, r1), o2 = fmaf((float)k, gf.m10, r2);
-                        if (gf.sin_on) {
+                        if (ser >= 0 || gf.sin_on) {
                             const float l = fmaf((float)k, dl, l0), mm = fmaf((float)k, dn, n0);
                             const float rr = l * l + mm * mm;
                             float np = 0.0f;
-                            if (gf.series) {
+                            if (ser == 1 || (ser < 0 && gf.series)) {
                                 // every source of the image has r^2 <= 0.01 (checked in k_reset)
                                 np = rr * fmaf(rr, fmaf(rr, fmaf(rr, 0.0390625f, 0.0625f), 0.125f), 0.5f);
                             } else if (rr <= 1.0f) {
@@ -1199,7 +1235,7 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
                             const float p01 = t0 * wm1, p11 = t1 * wm1;
                             const float p00 = t0 - p01, p10 = t1 - p11;
                             int* p0 = reinterpret_cast<int*>(reinterpret_cast<char*>(acc) +
-                                                             (gb + (unsigned)(__mul24(f2, 4 * A) + 4 * f1)));
+                                                             (unsigned)mad_i24(f2, 4 * A, (int)gb + 4 * f1));
                             acc_exact(p0, p00);
                             acc_exact(p0 + A, p01);
                             acc_exact(p0 + 1, p10);
@@ -1208,26 +1244,46 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
                     }
                 }
             };
-            if (unbounded)
-                sources(std::true_type{}, std::true_type{});
-            else if (cedge || redge)
-                sources(std::true_type{}, std::false_type{});
-            else
-                sources(std::false_type{}, std::false_type{});
+            // Bounded maps take the SIN form as a compile-time choice: 2-5%
+            // faster than the per-source test of the flags.
+            using Series = std::integral_constant<int, 1>;
+            using Closed = std::integral_constant<int, 0>;
+            if (unbounded) {
+                sources(std::true_type{}, std::true_type{}, std::integral_constant<int, -1>{});
+            } else if (cedge || redge) {
+                if (gf.series)
+                    sources(std::true_type{}, std::false_type{}, Series{});
+                else
+                    sources(std::true_type{}, std::false_type{}, Closed{});
+            } else {
+                if (gf.series)
+                    sources(std::false_type{}, std::false_type{}, Series{});
+                else
+                    sources(std::false_type{}, std::false_type{}, Closed{});
+            }
             __syncthreads();   // crow is rewritten by the next window
         }
         __syncthreads();   // rlo/rcnt are rewritten by the next row batch
     }
     float* out = img + (size_t)s * I * I;
     float lmax = 0.0f;
-    // Four consecutive pixels per thread per pass, one 16-byte store.
-    for (int q = tid; q < TS * TS / 4; q += NT) {
-        const int rr = q / (TS / 4), cc = 4 * (q % (TS / 4));
+    // Four consecutive pixels per thread per pass, one 16-byte store; each
+    // pass covers NT / (TS / 4) rows, and the pointers step by that.
+    {
+        constexpr int RSTEP = NT / (TS / 4);
+        static_assert((TS * TS / 4) % NT == 0, "whole passes");
+        const int rr = tid / (TS / 4), cc = 4 * (tid % (TS / 4));
         const int* a = acc + (rr + 1) * A + cc + 1;
-        const float4 v = make_float4((float)a[0] * iscale, (float)a[1] * iscale, (float)a[2] * iscale,
-                                     (float)a[3] * iscale);
-        *reinterpret_cast<float4*>(out + (size_t)(R0 + rr) * I + C0 + cc) = v;
-        lmax = fmaxf(lmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+        float* o = out + (size_t)(R0 + rr) * I + C0 + cc;
+#pragma unroll
+        for (int q = 0; q < TS * TS / 4 / NT; ++q) {
+            const float4 v = make_float4((float)a[0] * iscale, (float)a[1] * iscale, (float)a[2] * iscale,
+                                         (float)a[3] * iscale);
+            *reinterpret_cast<float4*>(o) = v;
+            lmax = fmaxf(lmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+            a += RSTEP * A;
+            o += (size_t)RSTEP * I;
+        }
     }
     const unsigned bmx = block_max_u<NT>(__float_as_uint(lmax), shm);
     if (tid == 0) atomicMax(max_slot(pmax, s, 1, (int)(blockIdx.x % kMaxSlots)), bmx);
