@@ -65,14 +65,18 @@ def rmax_of(B, cell, g):
     return int(max(hi.max(), (-lo).max()))
 
 
-def stage_bytes(nbl, image, g, rmax, tile, nsnap=3, nwin=1):
+def stage_bytes(nbl, image, g, rmax, tile, nsnap=3, nwin=1, fused=False):
     """Algorithmic HBM bytes per launch of each stage: every input read once and
     every output written once, for the nsnap snapshots (nwin windows) one launch
     processes.  Visibility records are 24 B (two bucket/offset pairs + complex
-    value), the column-transformed grid holds rmax+1 complex columns of I rows."""
+    value), the column-transformed grid holds rmax+1 complex columns of I rows.
+    fused: the last snapshot's splat carries the trigger (splat_trigger reads
+    its corrected image and the window's first two images, and writes three
+    terms per trigger tile; the trigger stage turns the terms into the tLISI)."""
     I = image
     kcols = rmax + 1
-    return {
+    un2 = (I // tile) ** 2
+    b = {
         "reset": nsnap * (g * 4 + 64),
         "bin": nsnap * nbl * (4 + 4 + 8 + 24 + 8),
         "scan": nsnap * (g * 4 + (g + 1) * 4),
@@ -80,8 +84,13 @@ def stage_bytes(nbl, image, g, rmax, tile, nsnap=3, nwin=1):
         "grid_colfft": nsnap * (nbl * 24 + kcols * I * 8),
         "row_c2r": nsnap * (kcols * I * 8 + I * I * 4),
         "splat": nsnap * (I * I * 4 * 2),
-        "trigger": nwin * (3 * I * I * 4 + (I // tile) ** 2 * 4),
+        "trigger": nwin * (3 * I * I * 4 + un2 * 4),
     }
+    if fused:
+        b["splat"] = (nsnap - 1) * (I * I * 4 * 2)
+        b["splat_trigger"] = I * I * 4 + 2 * I * I * 4 + un2 * 12
+        b["trigger"] = un2 * (12 + 4)
+    return b
 
 
 def load_traffic(stage: str):
@@ -295,8 +304,8 @@ def main() -> int:
         step()
     F.set_profiling(False)
     st = F.stage_times()
-    per = {k: v[0] / max(v[1], 1) for k, v in st.items()}
-    sb = stage_bytes(nbl, I, g, rmax, T)
+    per = {k: v[0] / v[1] for k, v in st.items() if v[1] > 0}
+    sb = stage_bytes(nbl, I, g, rmax, T, fused="splat_trigger" in per)
     stages = {k: {"ms": round(per[k], 5), "GB_s": round(sb[k] / (per[k] * 1e-3) / 1e9, 1) if per[k] > 0 else None,
                   "traffic_bytes": load_traffic(k)} for k in per}
     dom = max(per, key=lambda k: per[k])

@@ -86,6 +86,8 @@ int pipe_host(const void* vis_re, const void* vis_im, int vt, const float* bin, 
     char* hr = hs + S * region;
     const Tuning tu = C->tuning(image);
     const WorkView wall = C->work.view(nbl);
+    // the third snapshot's splat carries the trigger where it can
+    const bool fuse = splat_trigger_fusable(P->t, tu, (int)tile);
     auto enqueue = [&]() -> int {
         for (int s = 0; s < S; ++s) {
             const size_t sv = (size_t)s * nbl;
@@ -107,11 +109,15 @@ int pipe_host(const void* vis_re, const void* vis_im, int vt, const float* bin, 
             InputView in{d, d + o_vi, reinterpret_cast<const float*>(d + o_b), reinterpret_cast<const float*>(d + o_v),
                          nullptr, nbl, (int)nbl, vt};
             if (int rc = imaging_sequence(C, P->t, snapshot_view(wall, s, P->t), in, 1, (int)nbl, cell, nullptr,
-                                          (int)nbl, tu, st))
+                                          (int)nbl, tu, st, fuse && s == S - 1))
                 return rc;
         }
-        if (int rc = run_trigger(*C, *P, C->work, nbl, C->win.as<int>(), 1, (int)tile, C->out.as<float>(), st))
+        if (fuse) {
+            if (int rc = fused_trigger_sequence(C, P->t, wall, (int)tile, C->out.as<float>(), st)) return rc;
+        } else if (int rc = run_trigger(*C, *P, C->work, nbl, C->win.as<int>(), 1, (int)tile, C->out.as<float>(),
+                                        st)) {
             return rc;
+        }
         FIP_HIP(hipMemcpyAsync(hr, C->out.p, rbytes, hipMemcpyDeviceToHost, st));
         FIP_HIP(hipMemcpyAsync(hr + rbytes, C->work.meta.p, S * sizeof(SnapMeta), hipMemcpyDeviceToHost, st));
         return C->mark_work(st);
@@ -217,10 +223,8 @@ int fip_pipe_device(const float* vis_re, const float* vis_im, const float* bin, 
         if (int rc = C->ensure_win012()) return rc;
         if (int rc = C->order_work(st)) return rc;
         InputView in{vis_re, vis_im, bin, vin, nullptr, num_baselines, (int)num_baselines, FIP_VIS_F32};
-        if (int rc = run_imaging(*C, *P, C->work, in, 3, (int)num_baselines, cell_size, nullptr,
-                                 (int)num_baselines, C->tuning(image_size), st))
-            return rc;
-        if (int rc = run_trigger(*C, *P, C->work, num_baselines, C->win.as<int>(), 1, (int)unit_size, result, st))
+        if (int rc = pipe_sequence(C, P->t, C->work.view(num_baselines), in, (int)num_baselines, cell_size,
+                                   C->tuning(image_size), C->win.as<int>(), (int)unit_size, result, st))
             return rc;
         if (int rc = C->mark_work(st)) return rc;
         if (C->profiling) {

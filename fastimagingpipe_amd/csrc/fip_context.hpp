@@ -132,11 +132,22 @@ private:
 WorkView snapshot_view(const WorkView& w, int s, const PlanTables& p);
 
 // Enqueue reset..splat / the trigger on explicit tables and buffers; C (may
-// be null) only adds the profiling events.
+// be null) only adds the profiling events.  With defer_last the splat of the
+// view's last snapshot is left to fused_trigger_sequence.
 int imaging_sequence(Context* C, const PlanTables& p, const WorkView& w, const InputView& in, int S, int nmax,
-                     float cell, const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st);
+                     float cell, const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st,
+                     bool defer_last = false);
 int trigger_sequence(Context* C, const PlanTables& p, const WorkView& w, const int* win_dev, int nwin, int tile,
                      float* out_dev, hipStream_t st);
+// The fused tail of window (0, 1, 2) of a three-snapshot view whose last
+// splat imaging_sequence deferred (splat_trigger_fusable holds).
+int fused_trigger_sequence(Context* C, const PlanTables& p, const WorkView& w, int tile, float* out_dev,
+                           hipStream_t st);
+// FIpipe's three snapshots to the tLISI of window (0, 1, 2) (win_dev holds
+// it): the imaging, then the fused tail where splat_trigger_fusable holds,
+// else the last splat and the trigger.
+int pipe_sequence(Context* C, const PlanTables& p, const WorkView& w, const InputView& in, int nmax, float cell,
+                  const Tuning& tu, const int* win_dev, int tile, float* out_dev, hipStream_t st);
 
 // Enqueue reset..splat for S snapshots described by `in` into `w`.
 int run_imaging(Context& C, const Plan& P, Work& W, const InputView& in, int S, int nmax, float cell,

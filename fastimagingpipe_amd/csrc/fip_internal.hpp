@@ -129,6 +129,7 @@ enum Stage {
     kStGridFft,
     kStRowC2R,
     kStSplat,
+    kStSplatTrig,
     kStTrigger,
     kStCount
 };
@@ -145,8 +146,21 @@ hipError_t launch_stage1(const WorkView& w, const PlanTables& p, int S, const Tu
                          hipStream_t st);
 hipError_t launch_stage2(const WorkView& w, const PlanTables& p, int S, const Tuning& tu,
                          hipStream_t st);
+// Splat of snapshots 0..S-1 of the view.
 hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tuning& tu,
                         hipStream_t st);
+
+// Fused tail of one window (s1, s2, s3) with a 32-pixel trigger tile: the
+// splat of s3 computes the trigger terms from the images of s1 and s2
+// (already splatted) and its own values, without storing its image; then the
+// finish pass turns the terms into the tLISI.  The same bits as
+// launch_splat of s3 followed by launch_trigger.  Fusable when the splat
+// uses 64-pixel tiles and the trigger tile is 32.
+bool splat_trigger_fusable(const PlanTables& p, const Tuning& tu, int tile);
+hipError_t launch_splat_trigger(const WorkView& w, const PlanTables& p, int s1, int s2, int s3, hipStream_t st);
+hipError_t launch_trigger_finish(const WorkView& w, const PlanTables& p, int s1, int s2, int s3, int tile, float* out,
+                                 hipStream_t st);
+
 // Trigger over nwin windows; win: device int[3*nwin] of snapshot slots in img.
 hipError_t launch_trigger(const WorkView& w, const PlanTables& p, const int* win, int nwin,
                           int tile, float* out, hipStream_t st);

@@ -933,6 +933,39 @@ __device__ __forceinline__ void acc_exact(int* a, float x)
     atomicAdd(a, cvt_round_half_up_i32(x));
 }
 
+// Non-temporal 16-byte load for the trigger, which reads each image byte
+// once.  Measured on the one-window trigger: 157 -> 128 us at 8192^2 (three
+// images of 268 MB), unchanged at 4096^2; same bits.
+typedef float fip_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_stream(const float* p)
+{
+    const fip_f4 v = __builtin_nontemporal_load(reinterpret_cast<const fip_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Trigger terms of one pixel (FIPkernels.cu:499-552): dd = ||a - b| - |b - c||,
+// its sum and maximum, and the sum of the guarded ratio min(dd / b', 1) with
+// b' = b, or 1e-6 where b <= 0 (FIPkernels.cu:811).  Every trigger form adds
+// a lane's pixels in the same order through this function, so their sums
+// agree bit for bit.
+__device__ __forceinline__ void tlisi_px(float a, float b, float c, float& sum, float& mx, float& sr)
+{
+    const float dd = fabsf(fabsf(a - b) - fabsf(b - c));
+    const float b2 = (b <= 0.0f) ? 1e-6f : b;
+    sum += dd;
+    mx = fmaxf(mx, dd);
+    const float ratio = dd / b2;
+    sr += (ratio < 1.0f) ? ratio : 1.0f;
+}
+
+// tLISI of a T x T tile from its reduced terms and the window's maximum m3
+// (FIPkernels.cu:550).
+__device__ __forceinline__ float tlisi_value(float sum, float mx, float sr, float m3, int T)
+{
+    const float Tf = (float)T;
+    return 1.0f - (sum / Tf / Tf) * mx * (sr / Tf / Tf) / m3 / m3;
+}
+
 // Per target tile: bounding box and per-row ranges of the source pixels whose
 // bilinear footprint can reach the tile, then the forward splat
 // (finalinterp, FIPkernels.cu:378-413) accumulated in an LDS tile with a
@@ -942,11 +975,20 @@ __device__ __forceinline__ void acc_exact(int* a, float x)
 // lane finds its row from a table of the rows in which each run of 64
 // numbers starts (usually no further step) and keeps four independent
 // loads and position chains in flight; at most 64 VGPRs keep eight waves
-// per SIMD resident.
-template <int TS>
+// per SIMD resident.  Snapshot s0 + blockIdx.y.
+//
+// TRIG: the last snapshot of a three-snapshot window with the trigger fused
+// in.  Its image is not stored; each wave takes one 32 x 32 trigger tile of
+// the 64 x 64 target tile, reads the window's first two images (img_a,
+// img_b, written by an earlier launch) and this tile's values from LDS, and
+// stores the tile's reduced terms (sum, max, ratio sum) to part; the tLISI
+// follows once every image maximum is known (k_trigger_finish).
+template <int TS, bool TRIG>
 __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre, const SnapMeta* __restrict__ meta,
                                                const SnapGeom* __restrict__ geom, unsigned* __restrict__ pmax,
-                                               float* __restrict__ img, int I)
+                                               float* __restrict__ img, int I, int s0,
+                                               const float* __restrict__ img_a, const float* __restrict__ img_b,
+                                               float* __restrict__ part)
 {
     constexpr int NT = 256;
     constexpr int NW = NT / 64;           // waves
@@ -963,7 +1005,7 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
     __shared__ int bb[4];
     __shared__ unsigned shm[NT / 64];
     __shared__ unsigned smaxp;
-    const int s = blockIdx.y;
+    const int s = s0 + (int)blockIdx.y;
     const int ntx = I / TS;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int R0 = (tile / ntx) * TS, C0 = (tile % ntx) * TS;
@@ -1279,7 +1321,7 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
         for (int q = 0; q < TS * TS / 4 / NT; ++q) {
             const float4 v = make_float4((float)a[0] * iscale, (float)a[1] * iscale, (float)a[2] * iscale,
                                          (float)a[3] * iscale);
-            *reinterpret_cast<float4*>(o) = v;
+            if constexpr (!TRIG) *reinterpret_cast<float4*>(o) = v;
             lmax = fmaxf(lmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
             a += RSTEP * A;
             o += (size_t)RSTEP * I;
@@ -1287,19 +1329,70 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
     }
     const unsigned bmx = block_max_u<NT>(__float_as_uint(lmax), shm);
     if (tid == 0) atomicMax(max_slot(pmax, s, 1, (int)(blockIdx.x % kMaxSlots)), bmx);
+    if constexpr (TRIG) {
+        // The trigger tile of this wave; lanes take the pixels, and add them
+        // in the order, of k_trigger (16-byte groups lane + 64 u), so the
+        // reduced terms are the same bits as k_trigger's.
+        static_assert(TS == 64, "four 32 x 32 trigger tiles per workgroup, one per wave");
+        constexpr int T = 32;
+        const int tr = R0 + T * (wid >> 1), tc = C0 + T * (wid & 1);
+        float sum = 0.0f, mx = 0.0f, sr = 0.0f;
+        float4 av[4], bv[4], cv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = lane + 64 * u, row = p / (T / 4), col = 4 * (p % (T / 4));
+            const size_t q = (size_t)(tr + row) * I + (size_t)(tc + col);
+            av[u] = load_stream(img_a + q);
+            bv[u] = load_stream(img_b + q);
+            // the values this launch would have stored
+            const int* ac = acc + (tr - R0 + row + 1) * A + (tc - C0 + col + 1);
+            cv[u] = make_float4((float)ac[0] * iscale, (float)ac[1] * iscale, (float)ac[2] * iscale,
+                                (float)ac[3] * iscale);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            tlisi_px(av[u].x, bv[u].x, cv[u].x, sum, mx, sr);
+            tlisi_px(av[u].y, bv[u].y, cv[u].y, sum, mx, sr);
+            tlisi_px(av[u].z, bv[u].z, cv[u].z, sum, mx, sr);
+            tlisi_px(av[u].w, bv[u].w, cv[u].w, sum, mx, sr);
+        }
+        sum = wave_sum_f(sum);
+        mx = wave_max_f(mx);
+        sr = wave_sum_f(sr);
+        if (lane == 0) {
+            float* pp = part + 3 * ((size_t)(tr / T) * (I / T) + (size_t)(tc / T));
+            pp[0] = sum;
+            pp[1] = mx;
+            pp[2] = sr;
+        }
+    }
+}
+
+// tLISI of one window from the fused splat's reduced terms: each wave reduces
+// the window's three image maxima from their spread slots (as k_trigger does),
+// then each thread finishes one tile.
+__global__ __launch_bounds__(256) void k_trigger_finish(const float* __restrict__ part, SnapMeta* __restrict__ meta,
+                                                        unsigned* __restrict__ pmax, int s1, int s2, int s3, int I,
+                                                        int T, float* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned m1b = wave_max_u(*max_slot(pmax, s1, 1, lane)), m2b = wave_max_u(*max_slot(pmax, s2, 1, lane)),
+                   m3b = wave_max_u(*max_slot(pmax, s3, 1, lane));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the reduced maxima for host readers
+        meta[s1].maxd_bits = m1b;
+        meta[s2].maxd_bits = m2b;
+        meta[s3].maxd_bits = m3b;
+    }
+    const int un = I / T;
+    const int tile = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (tile >= un * un) return;
+    const float* pp = part + 3 * (size_t)tile;
+    const float m3 = fmaxf(__uint_as_float(m1b), fmaxf(__uint_as_float(m2b), __uint_as_float(m3b)));
+    out[tile] = tlisi_value(pp[0], pp[1], pp[2], m3, T);
 }
 
 // --------------------------------------------------------------- trigger
-
-// Non-temporal 16-byte load for the trigger, which reads each image byte
-// once.  Measured on the one-window trigger: 157 -> 128 us at 8192^2 (three
-// images of 268 MB), unchanged at 4096^2; same bits.
-typedef float fip_f4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ float4 load_stream(const float* p)
-{
-    const fip_f4 v = __builtin_nontemporal_load(reinterpret_cast<const fip_f4*>(p));
-    return make_float4(v.x, v.y, v.z, v.w);
-}
 
 // One wave per tile (four tiles per workgroup): per-lane partial sums over
 // 16-byte loads, then wave reductions only (FIPkernels.cu:499-552).
@@ -1320,16 +1413,8 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
     const float* d1 = img + (size_t)s1 * I * I;
     const float* d2 = img + (size_t)s2 * I * I;
     const float* d3 = img + (size_t)s3 * I * I;
-    const float C = 1e-6f;   // FIPkernels.cu:811
     float sum = 0.0f, mx = 0.0f, sr = 0.0f;
-    auto px = [&](float a, float b, float c) {
-        const float dd = fabsf(fabsf(a - b) - fabsf(b - c));
-        const float b2 = (b <= 0.0f) ? C : b;
-        sum += dd;
-        mx = fmaxf(mx, dd);
-        const float ratio = dd / b2;
-        sr += (ratio < 1.0f) ? ratio : 1.0f;
-    };
+    auto px = [&](float a, float b, float c) { tlisi_px(a, b, c, sum, mx, sr); };
     if ((T & 3) == 0) {
         // 16-byte loads: rows of a tile start at multiples of 4 floats; the
         // loads of four passes are issued together.
@@ -1374,9 +1459,7 @@ __global__ __launch_bounds__(256) void k_trigger(const float* __restrict__ img, 
     }
     if (lane == 0) {
         const float m3 = fmaxf(__uint_as_float(m1b), fmaxf(__uint_as_float(m2b), __uint_as_float(m3b)));
-        const float Tf = (float)T;
-        // FIPkernels.cu:550
-        out[(size_t)w * un * un + tile] = 1.0f - (sum / Tf / Tf) * mx * (sr / Tf / Tf) / m3 / m3;
+        out[(size_t)w * un * un + tile] = tlisi_value(sum, mx, sr, m3, T);
     }
 }
 
@@ -1401,18 +1484,10 @@ __global__ __launch_bounds__(256) void k_trigger_run(const float* __restrict__ i
     unsigned md[NI];
 #pragma unroll
     for (int j = 0; j < NI; ++j) md[j] = wave_max_u(j < ni ? *max_slot(pmax, i0 + j, 1, lane) : 0u);
-    const float C = 1e-6f;   // FIPkernels.cu:811
     float sum[K], mx[K], sr[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) sum[k] = mx[k] = sr[k] = 0.0f;
-    auto px = [&](int k, float a, float b, float c) {
-        const float dd = fabsf(fabsf(a - b) - fabsf(b - c));
-        const float b2 = (b <= 0.0f) ? C : b;
-        sum[k] += dd;
-        mx[k] = fmaxf(mx[k], dd);
-        const float ratio = dd / b2;
-        sr[k] += (ratio < 1.0f) ? ratio : 1.0f;
-    };
+    auto px = [&](int k, float a, float b, float c) { tlisi_px(a, b, c, sum[k], mx[k], sr[k]); };
     const size_t plane = (size_t)I * I;
     const float* base = img + (size_t)i0 * plane;
     if ((T & 3) == 0) {
@@ -1449,14 +1524,12 @@ __global__ __launch_bounds__(256) void k_trigger_run(const float* __restrict__ i
         // the reduced maxima for host readers
         for (int j = 0; j < ni; ++j) meta[i0 + j].maxd_bits = md[j];
     }
-    const float Tf = (float)T;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const float sk = wave_sum_f(sum[k]), mk = wave_max_f(mx[k]), rk = wave_sum_f(sr[k]);
         if (lane == 0 && k < nk) {
             const float m3 = fmaxf(__uint_as_float(md[k]), fmaxf(__uint_as_float(md[k + 1]), __uint_as_float(md[k + 2])));
-            // FIPkernels.cu:550
-            out[(size_t)(w0 + k) * un * un + tile] = 1.0f - (sk / Tf / Tf) * mk * (rk / Tf / Tf) / m3 / m3;
+            out[(size_t)(w0 + k) * un * un + tile] = tlisi_value(sk, mk, rk, m3, T);
         }
     }
 }
@@ -1465,8 +1538,8 @@ __global__ __launch_bounds__(256) void k_trigger_run(const float* __restrict__ i
 
 const char* stage_name(int st)
 {
-    static const char* names[kStCount] = {"reset", "bin", "scan", "scatter",
-                                          "grid_colfft", "row_c2r", "splat", "trigger"};
+    static const char* names[kStCount] = {"reset",   "bin",   "scan",          "scatter", "grid_colfft",
+                                          "row_c2r", "splat", "splat_trigger", "trigger"};
     return (st >= 0 && st < kStCount) ? names[st] : "?";
 }
 
@@ -1610,13 +1683,44 @@ hipError_t launch_stage2(const WorkView& w, const PlanTables& p, int S, const Tu
 
 hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tuning& tu, hipStream_t st)
 {
+    if (S <= 0) return hipSuccess;
     if (tu.splat_tile >= 64 && p.I % 64 == 0) {
         const int nt = (p.I / 64) * (p.I / 64);
-        hipLaunchKernelGGL(k_splat<64>, dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img, p.I);
+        hipLaunchKernelGGL((k_splat<64, false>), dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img,
+                           p.I, 0, nullptr, nullptr, nullptr);
     } else {
         const int nt = (p.I / 32) * (p.I / 32);
-        hipLaunchKernelGGL(k_splat<32>, dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img, p.I);
+        hipLaunchKernelGGL((k_splat<32, false>), dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img,
+                           p.I, 0, nullptr, nullptr, nullptr);
     }
+    return hipGetLastError();
+}
+
+bool splat_trigger_fusable(const PlanTables& p, const Tuning& tu, int tile)
+{
+    return tu.splat_tile >= 64 && p.I % 64 == 0 && tile == 32;
+}
+
+// The reduced trigger terms go to the K tiles' buffer, which the column and
+// row passes are done with (3 floats per trigger tile; the buffer holds
+// 8 I (g/2 + 1) bytes per snapshot).
+static float* trigger_terms(const WorkView& w) { return reinterpret_cast<float*>(w.kt); }
+
+hipError_t launch_splat_trigger(const WorkView& w, const PlanTables& p, int s1, int s2, int s3, hipStream_t st)
+{
+    const int nt = (p.I / 64) * (p.I / 64);
+    const size_t plane = (size_t)p.I * p.I;
+    hipLaunchKernelGGL((k_splat<64, true>), dim3(nt, 1), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img, p.I,
+                       s3, w.img + (size_t)s1 * plane, w.img + (size_t)s2 * plane, trigger_terms(w));
+    return hipGetLastError();
+}
+
+hipError_t launch_trigger_finish(const WorkView& w, const PlanTables& p, int s1, int s2, int s3, int tile, float* out,
+                                 hipStream_t st)
+{
+    const int un = p.I / tile;
+    hipLaunchKernelGGL(k_trigger_finish, dim3((un * un + 255) / 256), dim3(256), 0, st, trigger_terms(w), w.meta,
+                       w.pmax, s1, s2, s3, p.I, tile, out);
     return hipGetLastError();
 }
 

@@ -192,11 +192,8 @@ extern "C" int fip_pipe_device_noalloc(const float* vis_re, const float* vis_im,
         w.stride = num_baselines;
         hipStream_t st = static_cast<hipStream_t>(stream);
         InputView in{vis_re, vis_im, bin, vin, nullptr, num_baselines, (int)num_baselines, FIP_VIS_F32};
-        if (int rc = imaging_sequence(nullptr, p, w, in, 3, (int)num_baselines, cell_size, nullptr,
-                                      (int)num_baselines, default_tuning(image_size), st))
-            return rc;
-        return trigger_sequence(nullptr, p, w, reinterpret_cast<const int*>(base + L.win), 1, (int)unit_size, result,
-                                st);
+        return pipe_sequence(nullptr, p, w, in, (int)num_baselines, cell_size, default_tuning(image_size),
+                             reinterpret_cast<const int*>(base + L.win), (int)unit_size, result, st);
     } catch (...) {
         return set_error(FIP_ERR_HIP, "unexpected exception");
     }

@@ -35,8 +35,8 @@ from collections import defaultdict
 # event timing reports).
 STAGES = [
     (r"k_reset", "reset"), (r"k_bin", "bin"), (r"k_scan", "scan"), (r"k_scatter", "scatter"),
-    (r"k_stage1", "grid_colfft"), (r"k_stage2", "row_c2r"), (r"k_splat", "splat"),
-    (r"k_trigger", "trigger"),
+    (r"k_stage1", "grid_colfft"), (r"k_stage2", "row_c2r"), (r"k_splat<\d+, true>", "splat_trigger"),
+    (r"k_splat", "splat"), (r"k_trigger", "trigger"),
 ]
 COPY_PATTERN = r"copyBuffer|Copy"
 

@@ -55,6 +55,28 @@ def test_pipe_deterministic(fip):
     assert np.array_equal(i1.view(np.uint32), i2.view(np.uint32))
 
 
+def test_fused_splat_trigger_equals_image_then_trigger(fip):
+    """With a 32-pixel trigger tile and 64-pixel splat tiles the third
+    snapshot's splat carries the trigger, and its image is never stored.  The
+    tLISI must be the same bits as the unfused routes: fip_image (all three
+    images stored) then fip_trigger on them, and FIpipe with 32-pixel splat
+    tiles, which does not fuse (the exact integer splat gives the same images
+    for either tile size)."""
+    image, nbl, cell, tile = 1024, 8000, 1.45e-5, 32
+    vr, vi, B, V = snapshots(21, 3, nbl, image, cell)
+    fused = fip.fipipe(vr, vi, B, V, nbl, image, 3, cell, tile)
+    _, img, mx = fip.image(vr, vi, B, V, nbl, image, 3, cell)
+    ref = fip.trigger(img[0], img[1], img[2], mx, tile)
+    print("tLISI min", fused.min(), "max |fused - image+trigger|", np.abs(fused - ref).max())
+    assert np.array_equal(fused.view(np.uint32), ref.view(np.uint32))
+    fip.set_tuning(image, 2, 4, 32, 0)
+    try:
+        unfused = fip.fipipe(vr, vi, B, V, nbl, image, 3, cell, tile)
+    finally:
+        fip.set_tuning(image, 2, 4, 64, 0)   # the defaults at 1024^2
+    assert np.array_equal(fused.view(np.uint32), unfused.view(np.uint32))
+
+
 def test_trigger_matches_oracle(fip, oracle):
     rng = np.random.default_rng(3)
     I, T = 256, 16
