@@ -211,27 +211,27 @@ def main(argv=None) -> int:
         return 0
     if args.cmd == "stats" and args.last > 0:
         rows = kernel_trace_last(args.dir, args.last)
-        print(f"{'kernel':<72} {'stage':<12} {'calls':>6} {'avg_us':>10} {'median_us':>10} {'min_us':>8} "
+        print(f"{'kernel':<72} {'stage':<13} {'calls':>6} {'avg_us':>10} {'median_us':>10} {'min_us':>8} "
               f"{'max_us':>8} {'pct':>6}")
         for r in rows:
-            print(f"{short(r['name']):<72} {stage_of(r['name']) or '-':<12} {r['calls']:>6} "
+            print(f"{short(r['name']):<72} {stage_of(r['name']) or '-':<13} {r['calls']:>6} "
                   f"{r['avg_ns'] / 1e3:>10.2f} {r['median_ns'] / 1e3:>10.2f} {r['min_ns'] / 1e3:>8.2f} "
                   f"{r['max_ns'] / 1e3:>8.2f} {r['pct']:>6.2f}")
         return 0
     if args.cmd == "stats":
         rows = kernel_stats(args.dir)
-        print(f"{'kernel':<72} {'stage':<12} {'calls':>6} {'avg_us':>10} {'total_ms':>10} {'pct':>6}")
+        print(f"{'kernel':<72} {'stage':<13} {'calls':>6} {'avg_us':>10} {'total_ms':>10} {'pct':>6}")
         for r in rows:
-            print(f"{short(r['name']):<72} {stage_of(r['name']) or '-':<12} {r['calls']:>6} "
+            print(f"{short(r['name']):<72} {stage_of(r['name']) or '-':<13} {r['calls']:>6} "
                   f"{r['avg_ns'] / 1e3:>10.2f} {r['total_ns'] / 1e6:>10.3f} {r['pct']:>6.2f}")
         return 0
     t = traffic(args.fetch_dir, args.write_dir, args.copy_bytes)
     c = t["calibration"]
     print(f"copy calibration: {c['copy_bytes']} B copied -> FETCH_SIZE {c['fetch_kib']:.0f} KiB "
           f"(x{c['fetch_factor']:.4f}), WRITE_SIZE {c['write_kib']:.0f} KiB (x{c['write_factor']:.4f})")
-    print(f"{'stage':<12} {'dispatches':>10} {'fetch_MB':>10} {'write_MB':>10} {'total_MB':>10}  kernel")
+    print(f"{'stage':<13} {'dispatches':>10} {'fetch_MB':>10} {'write_MB':>10} {'total_MB':>10}  kernel")
     for st, r in t["kernels"].items():
-        print(f"{st:<12} {r['dispatches']:>10} {r['fetch_bytes'] / 1e6:>10.2f} {r['write_bytes'] / 1e6:>10.2f} "
+        print(f"{st:<13} {r['dispatches']:>10} {r['fetch_bytes'] / 1e6:>10.2f} {r['write_bytes'] / 1e6:>10.2f} "
               f"{r['bytes'] / 1e6:>10.2f}  {short(r['kernel'], 50)}")
     if args.json:
         with open(args.json, "w") as f:
