@@ -346,10 +346,9 @@ extern "C" int fip_pipe_raw_device(const fip_raw_view* raw, size_t image_size, f
             return rc;
         InputView in{C->in_vr.p, C->in_vi.p, C->in_b.as<float>(), C->in_v.as<float>(), nullptr, ns, (int)ns,
                      FIP_VIS_F32};
-        if (int rc = run_imaging(*C, *P, C->work, in, S, (int)ns, cell_size, nullptr, (int)ns,
-                                 C->tuning(image_size), st))
+        if (int rc = pipe_sequence(C, P->t, C->work.view(ns), in, (int)ns, cell_size, C->tuning(image_size),
+                                   C->win.as<int>(), (int)unit_size, result, st))
             return rc;
-        if (int rc = run_trigger(*C, *P, C->work, ns, C->win.as<int>(), 1, (int)unit_size, result, st)) return rc;
         if (bin_out)
             FIP_HIP(hipMemcpyAsync(bin_out, C->in_b.p, S * ns * sizeof(float2), hipMemcpyDeviceToDevice, st));
         if (vin_out) FIP_HIP(hipMemcpyAsync(vin_out, C->in_v.p, S * 9 * sizeof(float), hipMemcpyDeviceToDevice, st));
