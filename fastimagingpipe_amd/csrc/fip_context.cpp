@@ -368,7 +368,8 @@ int trigger_sequence(Context* C, const PlanTables& p, const WorkView& w, const i
 int fused_trigger_sequence(Context* C, const PlanTables& p, const WorkView& w, int tile, float* out_dev,
                            hipStream_t st)
 {
-    FIP_STAGE(C, kStSplatTrig, st, launch_splat_trigger(w, p, 0, 1, 2, st));
+    const size_t plane = (size_t)p.I * p.I;
+    FIP_STAGE(C, kStSplatTrig, st, launch_splat_trigger(w, p, 2, w.img, w.img + plane, false, st));
     FIP_STAGE(C, kStTrigger, st, launch_trigger_finish(w, p, 0, 1, 2, tile, out_dev, st));
     return FIP_OK;
 }

@@ -150,14 +150,16 @@ hipError_t launch_stage2(const WorkView& w, const PlanTables& p, int S, const Tu
 hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tuning& tu,
                         hipStream_t st);
 
-// Fused tail of one window (s1, s2, s3) with a 32-pixel trigger tile: the
-// splat of s3 computes the trigger terms from the images of s1 and s2
-// (already splatted) and its own values, without storing its image; then the
-// finish pass turns the terms into the tLISI.  The same bits as
-// launch_splat of s3 followed by launch_trigger.  Fusable when the splat
-// uses 64-pixel tiles and the trigger tile is 32.
+// Fused tail of one window (a, b, s3) with a 32-pixel trigger tile: the
+// splat of snapshot s3 of the view computes the trigger terms from the
+// window's first two images img_a, img_b (already splatted) and its own
+// values, and stores its own image only if store; then the finish pass turns
+// the terms into the tLISI.  The same bits as launch_splat of s3 followed by
+// launch_trigger.  Fusable when the splat uses 64-pixel tiles and the
+// trigger tile is 32.  The terms live in w.kt, which the row pass is done with.
 bool splat_trigger_fusable(const PlanTables& p, const Tuning& tu, int tile);
-hipError_t launch_splat_trigger(const WorkView& w, const PlanTables& p, int s1, int s2, int s3, hipStream_t st);
+hipError_t launch_splat_trigger(const WorkView& w, const PlanTables& p, int s3, const float* img_a,
+                                const float* img_b, bool store, hipStream_t st);
 hipError_t launch_trigger_finish(const WorkView& w, const PlanTables& p, int s1, int s2, int s3, int tile, float* out,
                                  hipStream_t st);
 

@@ -986,17 +986,17 @@ __device__ __forceinline__ float tlisi_value(float sum, float mx, float sr, floa
 // per SIMD resident.  Snapshot s0 + blockIdx.y.
 //
 // TRIG: the last snapshot of a three-snapshot window with the trigger fused
-// in.  Its image is not stored; each wave takes one 32 x 32 trigger tile of
-// the 64 x 64 target tile, reads the window's first two images (img_a,
-// img_b, written by an earlier launch) and this tile's values from LDS, and
-// stores the tile's reduced terms (sum, max, ratio sum) to part; the tLISI
-// follows once every image maximum is known (k_trigger_finish).
+// in.  Its image is stored only if store_img; each wave takes one 32 x 32
+// trigger tile of the 64 x 64 target tile, reads the window's first two
+// images (img_a, img_b, written by an earlier launch) and this tile's values
+// from LDS, and stores the tile's reduced terms (sum, max, ratio sum) to part;
+// the tLISI follows once every image maximum is known (k_trigger_finish).
 template <int TS, bool TRIG>
 __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre, const SnapMeta* __restrict__ meta,
                                                const SnapGeom* __restrict__ geom, unsigned* __restrict__ pmax,
                                                float* __restrict__ img, int I, int s0,
                                                const float* __restrict__ img_a, const float* __restrict__ img_b,
-                                               float* __restrict__ part)
+                                               float* __restrict__ part, int store_img)
 {
     constexpr int NT = 256;
     constexpr int NW = NT / 64;           // waves
@@ -1329,7 +1329,7 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
         for (int q = 0; q < TS * TS / 4 / NT; ++q) {
             const float4 v = make_float4((float)a[0] * iscale, (float)a[1] * iscale, (float)a[2] * iscale,
                                          (float)a[3] * iscale);
-            if constexpr (!TRIG) *reinterpret_cast<float4*>(o) = v;
+            if (!TRIG || store_img) *reinterpret_cast<float4*>(o) = v;
             lmax = fmaxf(lmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
             a += RSTEP * A;
             o += (size_t)RSTEP * I;
@@ -1695,11 +1695,11 @@ hipError_t launch_splat(const WorkView& w, const PlanTables& p, int S, const Tun
     if (tu.splat_tile >= 64 && p.I % 64 == 0) {
         const int nt = (p.I / 64) * (p.I / 64);
         hipLaunchKernelGGL((k_splat<64, false>), dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img,
-                           p.I, 0, nullptr, nullptr, nullptr);
+                           p.I, 0, nullptr, nullptr, nullptr, 1);
     } else {
         const int nt = (p.I / 32) * (p.I / 32);
         hipLaunchKernelGGL((k_splat<32, false>), dim3(nt, S), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img,
-                           p.I, 0, nullptr, nullptr, nullptr);
+                           p.I, 0, nullptr, nullptr, nullptr, 1);
     }
     return hipGetLastError();
 }
@@ -1714,12 +1714,12 @@ bool splat_trigger_fusable(const PlanTables& p, const Tuning& tu, int tile)
 // 8 I (g/2 + 1) bytes per snapshot).
 static float* trigger_terms(const WorkView& w) { return reinterpret_cast<float*>(w.kt); }
 
-hipError_t launch_splat_trigger(const WorkView& w, const PlanTables& p, int s1, int s2, int s3, hipStream_t st)
+hipError_t launch_splat_trigger(const WorkView& w, const PlanTables& p, int s3, const float* img_a,
+                                const float* img_b, bool store, hipStream_t st)
 {
     const int nt = (p.I / 64) * (p.I / 64);
-    const size_t plane = (size_t)p.I * p.I;
     hipLaunchKernelGGL((k_splat<64, true>), dim3(nt, 1), dim3(256), 0, st, w.pre, w.meta, w.geom, w.pmax, w.img, p.I,
-                       s3, w.img + (size_t)s1 * plane, w.img + (size_t)s2 * plane, trigger_terms(w));
+                       s3, img_a, img_b, trigger_terms(w), store ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -417,15 +417,28 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     SRV_HIP(launch_scatter(w, p, 1, (int)N, st));
     SRV_HIP(launch_stage1(w, p, 1, ss.tu, st));
     SRV_HIP(launch_stage2(w, p, 1, ss.tu, st));
-    SRV_HIP(launch_splat(w, p, 1, ss.tu, st));
+    const int T = (int)ss.cfg.unit_size;
     if (sl->window >= 0) {
+        // The window (idx-2, idx-1, idx) in ring slots.
         WorkView wr = w;
         wr.meta = ss.meta.as<SnapMeta>();
         wr.pmax = ss.pmax.as<unsigned>();
         wr.img = ss.img.as<float>();
         float* dout = ss.out.as<float>();
-        SRV_HIP(launch_trigger(wr, p, ss.win.as<int>() + 3 * slot, 1, (int)ss.cfg.unit_size, dout, st));
+        if (splat_trigger_fusable(p, ss.tu, T)) {
+            // This snapshot's splat carries the trigger; its image is kept for
+            // the next two windows.
+            const int sa = (int)((idx - 2) % srv->ring), sb = (int)((idx - 1) % srv->ring);
+            SRV_HIP(launch_splat_trigger(w, p, 0, wr.img + (size_t)sa * I * I, wr.img + (size_t)sb * I * I, true,
+                                         st));
+            SRV_HIP(launch_trigger_finish(wr, p, sa, sb, slot, T, dout, st));
+        } else {
+            SRV_HIP(launch_splat(w, p, 1, ss.tu, st));
+            SRV_HIP(launch_trigger(wr, p, ss.win.as<int>() + 3 * slot, 1, T, dout, st));
+        }
         SRV_HIP(hipMemcpyAsync(h + L.res, dout, un * un * 4, hipMemcpyDeviceToHost, st));
+    } else {
+        SRV_HIP(launch_splat(w, p, 1, ss.tu, st));
     }
     {
         std::lock_guard<std::mutex> lk2(srv->mu);
