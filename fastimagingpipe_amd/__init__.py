@@ -129,6 +129,7 @@ _sig("fip_server_submit", [_vp, _i, _vp, _vp, _vp, _vp, ctypes.c_double])
 _sig("fip_server_drain", [_vp])
 _sig("fip_server_stats", [_vp, _vp, _vp, _vp, _vp, _vp])
 _sig("fip_server_latest", [_vp, _i, _vp, _vp])
+_sig("fip_server_sequences", [_vp, _vp])
 _sig("fip_server_destroy", [_vp], None)
 
 
@@ -439,6 +440,13 @@ class Server:
                                     ctypes.byref(mean), ctypes.byref(done), ctypes.byref(refused)))
         return {"p50_ms": p50.value, "p99_ms": p99.value, "mean_ms": mean.value,
                 "processed": done.value, "refused": refused.value}
+
+    def sequences(self) -> int:
+        """Launch sequences enqueued so far: snapshots that queued up behind a
+        running sequence of their stream were imaged together."""
+        n = ctypes.c_long()
+        _check(lib.fip_server_sequences(self._h, ctypes.byref(n)))
+        return n.value
 
     def latest(self, sid: int):
         s = self._streams[sid]

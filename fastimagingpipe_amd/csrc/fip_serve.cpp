@@ -1,15 +1,22 @@
 // Serving form: streaming imaging + trigger for several sub-array streams.
 //
-// Each stream owns a HIP stream (default priority), single-snapshot work
-// buffers, a device ring of `ring` image slots and its window table.  A
-// submitted snapshot reserves room for its inputs (and its pinned tLISI) in
-// an input pool shared by all streams, is copied there through the pool's
-// pinned mirror, imaged into the next ring slot and, from the third snapshot
-// on, triggered against the two previous ring slots (the window of
-// FIPkernels.cu:806-851); the tLISI is copied to pinned memory and an event
-// recorded behind it.  A completion thread polls the events of the
+// Each stream owns a HIP stream (default priority), work buffers for
+// kServeBatch snapshots, a device ring of `ring` image slots and its window
+// table.  A submitted snapshot reserves room for its inputs (and its pinned
+// tLISI) in an input pool shared by all streams and is copied there through
+// the pool's pinned mirror.  It is then imaged into its ring slot and, from
+// the third snapshot on, triggered against the two previous ring slots (the
+// window of FIPkernels.cu:806-851); the tLISI is copied to pinned memory and
+// an event recorded behind it.  A completion thread polls the events of the
 // snapshots in flight, stamps each completion time and releases the
 // reservation, so no host callback ever stalls a stream.
+//
+// Dynamic batching: a snapshot submitted while its stream's previous launch
+// sequence is still running waits in the stream's queue; when that sequence
+// completes, the completion thread launches the queued snapshots (up to
+// kServeBatch, in consecutive ring slots) as one multi-snapshot sequence.
+// An idle stream launches a snapshot at once, alone, so the latency at low
+// load is that of one snapshot.
 //
 // Admission by HBM budget: at creation every stream's fixed device state
 // (work buffers, ring, window table) plus one snapshot of the largest
@@ -48,6 +55,10 @@ size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 
 size_t esize(int vt) { return vt == FIP_VIS_F32 ? 4 : 2; }
 
+// Snapshots of one stream imaged together when they queue up behind a busy
+// stream (work buffers for this many are part of each stream's fixed state).
+constexpr int kServeBatch = 3;
+
 // Byte layout of one snapshot's reservation: vis_re | vis_im | bin | vin |
 // tLISI (the last only used in the pinned mirror).
 struct InLayout {
@@ -66,17 +77,57 @@ InLayout in_layout(const fip_stream_config& c)
     return L;
 }
 
-// Fixed device bytes of one stream: single-snapshot work buffers, the ring
-// of images with their metadata and max slots, window table and tLISI.
+// Inputs of up to kServeBatch queued snapshots gathered for one launch
+// sequence, in the InputView layout (stride N): vis_re[B][N] | vis_im[B][N] |
+// bin[B][N][2] | vin[B][9], each part 256-byte aligned.
+struct BatchLayout {
+    size_t vi, b, v, bytes;
+};
+
+BatchLayout batch_layout(const fip_stream_config& c)
+{
+    const size_t N = c.num_baselines, B = kServeBatch, e = esize(c.vis_type);
+    BatchLayout L;
+    L.vi = al256(B * N * e);
+    L.b = L.vi + al256(B * N * e);
+    L.v = L.b + al256(B * N * 2 * sizeof(float));
+    L.bytes = L.v + al256(B * 9 * sizeof(float));
+    return L;
+}
+
+// Fixed device bytes of one stream: work buffers and gathered inputs for
+// kServeBatch snapshots, the ring of images with their metadata and max
+// slots, the window table and the tLISI of kServeBatch windows.
 size_t stream_fixed_bytes(const fip_stream_config& c, int ring)
 {
-    const size_t I = c.image_size, N = c.num_baselines;
+    const size_t I = c.image_size, N = c.num_baselines, B = kServeBatch;
     const size_t G = grid_size_for(I), M = G / 2;
     const size_t un = I / c.unit_size;
     const size_t work = N * sizeof(VisRec) * 2 + N * sizeof(int2) + G * 4 + (G + 1) * 4 + I * (M + 1) * 8 + I * I * 4;
     const size_t ringb = (size_t)ring * (I * I * 4 + sizeof(SnapMeta) + sizeof(SnapGeom)) +
                          pmax_words(ring) * sizeof(unsigned) + 3 * sizeof(int) * (size_t)ring;
-    return work + ringb + un * un * 4 + 256;
+    return B * work + batch_layout(c).bytes + ringb + B * un * un * 4 + 256;
+}
+
+// One launch copies the queued snapshots' inputs from their reservations into
+// the batch layout, in 2-byte units (every part is a whole number of them).
+struct GatherPart {
+    const unsigned short* src;
+    unsigned short* dst;
+    unsigned n;
+};
+struct GatherJob {
+    GatherPart part[4 * kServeBatch];
+    int nparts;
+};
+
+__global__ __launch_bounds__(256) void k_gather_inputs(GatherJob job)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (int k = 0; k < job.nparts; ++k) {
+        const GatherPart pt = job.part[k];
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < pt.n; i += stride) pt.dst[i] = pt.src[i];
+    }
 }
 
 }  // namespace
@@ -89,7 +140,10 @@ struct InSlot {
     size_t off = 0, bytes = 0;   // reservation in the input pool
     hipEvent_t done = nullptr;   // recorded after the tLISI copy
     double arrival = 0.0;
+    long index = 0;              // position in its stream
+    int ring_slot = 0;           // index % ring: its image slot
     long window = -1;            // window index produced by this snapshot, -1 if none
+    bool last = false;           // last snapshot of its launch sequence
     int stream_id = 0;
     fip_server* srv = nullptr;
 };
@@ -99,10 +153,16 @@ struct SubStream {
     Plan* plan = nullptr;
     Tuning tu{};
     hipStream_t hs = nullptr;
-    DevBuf rec, bkslot, sorted, hist, bstart, kt, pre, work;   // one snapshot
+    DevBuf rec, bkslot, sorted, hist, bstart, kt, pre, work;   // kServeBatch snapshots
+    DevBuf batch_in;                                           // gathered inputs (BatchLayout)
     DevBuf img, meta, geom, pmax;                              // ring
     DevBuf win, out;
     InLayout lay{};
+    // Admitted snapshots waiting for the running launch sequence (guarded by
+    // the server's qmu), and whether one is running.
+    std::vector<InSlot*> queue;
+    bool running = false;
+    long sequences = 0;   // launch sequences enqueued (guarded by qmu)
     long submitted = 0;
     std::vector<float> latest;
     long latest_window = -1;
@@ -170,9 +230,11 @@ struct fip_server {
     // completion tracking
     std::mutex pmu;
     std::condition_variable pcv;
-    std::vector<InSlot*> pending;  // submitted, event not yet seen complete
+    std::vector<InSlot*> pending;  // launched, event not yet seen complete
     bool stop = false;
     std::thread poller;
+    std::mutex qmu;                // guards the streams' queues and running flags
+    int launch_error = 0;          // first failed launch of the completion thread (under mu)
 };
 
 namespace {
@@ -200,6 +262,167 @@ void on_complete(InSlot* sl, double t)
     srv->idle.push_back(sl);
 }
 
+// Completes the slots of a launch sequence that could not be enqueued: the
+// stream is synchronised first (copies out of the reservations may be in
+// flight), then each reservation is released and the first error kept.
+void fail_slots(fip_server* srv, SubStream& ss, const std::vector<InSlot*>& b, int rc)
+{
+    (void)hipStreamSynchronize(ss.hs);
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->inflight -= (long)b.size();
+        if (!srv->launch_error) srv->launch_error = rc;
+    }
+    std::lock_guard<std::mutex> lk(srv->amu);
+    for (InSlot* sl : b) {
+        srv->alloc.give(sl->off, sl->bytes);
+        srv->idle.push_back(sl);
+    }
+}
+
+// Enqueues the first queued snapshots of ss (up to kServeBatch, in
+// consecutive ring slots) as one launch sequence: their inputs gathered into
+// the batch layout (one snapshot is read from its reservation in place), the
+// imaging, the windows they complete, the tLISI copies and one event per
+// snapshot, which the completion thread then watches.  The caller holds the
+// context lock and srv->qmu, and ss has no sequence running.
+//
+// A sequence images all of its snapshots before its trigger reads the first
+// window, whose two earlier images sit in ring slots r0-2 and r0-1 (mod ring)
+// from earlier sequences; the sequence writes slots r0 .. r0+k-1, so k is at
+// most ring-2 to leave those two untouched, and at most ring-r0 so that its
+// slots are consecutive.
+int launch_queued(fip_server* srv, SubStream& ss)
+{
+    if (ss.queue.empty()) return FIP_OK;
+    const int ring = srv->ring;
+    const int r0 = ss.queue.front()->ring_slot;
+    const int k = std::min({(int)ss.queue.size(), kServeBatch, ring - r0, ring - 2});
+    std::vector<InSlot*> b(ss.queue.begin(), ss.queue.begin() + k);
+    ss.queue.erase(ss.queue.begin(), ss.queue.begin() + k);
+    auto fail = [&](int rc) {
+        fail_slots(srv, ss, b, rc);
+        return rc;
+    };
+#define SRV_HIP(x)                                              \
+    do {                                                        \
+        hipError_t e_ = (x);                                    \
+        if (e_ != hipSuccess) return fail(hip_error(e_, #x));   \
+    } while (0)
+    hipStream_t st = ss.hs;
+    const size_t N = ss.cfg.num_baselines, I = ss.cfg.image_size, plane = I * I;
+    const size_t vb = N * esize(ss.cfg.vis_type);
+    const size_t un = I / ss.cfg.unit_size;
+    const int T = (int)ss.cfg.unit_size;
+    const InLayout& L = ss.lay;
+    char* hbase = static_cast<char*>(srv->hpool.p);
+    char* dbase = srv->pool.as<char>();
+    WorkView w;
+    w.rec = ss.rec.as<VisRec>();
+    w.bkslot = ss.bkslot.as<int2>();
+    w.sorted = ss.sorted.as<VisRec>();
+    w.hist = ss.hist.as<int>();
+    w.bstart = ss.bstart.as<int>();
+    w.meta = ss.meta.as<SnapMeta>() + r0;
+    w.geom = ss.geom.as<SnapGeom>() + r0;
+    w.kt = ss.kt.as<float2>();
+    w.pre = ss.pre.as<float>();
+    w.img = ss.img.as<float>() + (size_t)r0 * plane;
+    w.work = ss.work.as<unsigned>();
+    w.pmax = ss.pmax.as<unsigned>() + pmax_index(r0, 0, 0);
+    w.stride = N;
+    InputView in;
+    if (k == 1) {
+        char* d = dbase + b[0]->off;
+        in = InputView{d, d + L.vi, reinterpret_cast<const float*>(d + L.b), reinterpret_cast<const float*>(d + L.v),
+                       nullptr, N, (int)N, ss.cfg.vis_type};
+    } else {
+        const BatchLayout BL = batch_layout(ss.cfg);
+        char* g = ss.batch_in.as<char>();
+        GatherJob job{};
+        for (int j = 0; j < k; ++j) {
+            const char* d = dbase + b[j]->off;
+            const struct {
+                const char* src;
+                char* dst;
+                size_t bytes;
+            } parts[4] = {{d, g + j * vb, vb},
+                          {d + L.vi, g + BL.vi + j * vb, vb},
+                          {d + L.b, g + BL.b + j * N * 8, N * 8},
+                          {d + L.v, g + BL.v + j * 36, 36}};
+            for (const auto& pt : parts)
+                job.part[job.nparts++] = GatherPart{reinterpret_cast<const unsigned short*>(pt.src),
+                                                    reinterpret_cast<unsigned short*>(pt.dst),
+                                                    (unsigned)(pt.bytes / 2)};
+        }
+        hipLaunchKernelGGL(k_gather_inputs, dim3(512), dim3(256), 0, st, job);
+        SRV_HIP(hipGetLastError());
+        in = InputView{g, g + BL.vi, reinterpret_cast<const float*>(g + BL.b), reinterpret_cast<const float*>(g + BL.v),
+                       nullptr, N, (int)N, ss.cfg.vis_type};
+    }
+    const PlanTables& p = ss.plan->t;
+    // One snapshot that completes a window carries the trigger in its splat
+    // (its image still stored for the next two windows).
+    const bool fuse = k == 1 && b[0]->window >= 0 && splat_trigger_fusable(p, ss.tu, T);
+    if (int rc = imaging_sequence(nullptr, p, w, in, k, (int)N, ss.cfg.cell_size, nullptr, (int)N, ss.tu, st, fuse))
+        return fail(rc);
+    WorkView wr = w;   // the whole ring
+    wr.meta = ss.meta.as<SnapMeta>();
+    wr.pmax = ss.pmax.as<unsigned>();
+    wr.img = ss.img.as<float>();
+    float* dout = ss.out.as<float>();
+    int j0 = 0;   // first snapshot of the sequence that completes a window
+    while (j0 < k && b[j0]->window < 0) ++j0;
+    if (fuse) {
+        const int sa = (int)((b[0]->index - 2) % ring), sb = (int)((b[0]->index - 1) % ring);
+        SRV_HIP(launch_splat_trigger(w, p, 0, wr.img + (size_t)sa * plane, wr.img + (size_t)sb * plane, true, st));
+        SRV_HIP(launch_trigger_finish(wr, p, sa, sb, r0, T, dout, st));
+    } else if (j0 < k) {
+        // windows (r-2, r-1, r) of ring slots r0+j0 .. r0+k-1, one table row each
+        SRV_HIP(launch_trigger(wr, p, ss.win.as<int>() + 3 * (r0 + j0), k - j0, T, dout, st));
+    }
+    for (int j = j0; j < k; ++j)
+        SRV_HIP(hipMemcpyAsync(hbase + b[j]->off + L.res, dout + (size_t)(j - j0) * un * un, un * un * 4,
+                               hipMemcpyDeviceToHost, st));
+    for (int j = 0; j < k; ++j) {
+        b[j]->last = j == k - 1;
+        SRV_HIP(hipEventRecord(b[j]->done, st));
+    }
+#undef SRV_HIP
+    ss.running = true;
+    ss.sequences += 1;
+    {
+        std::lock_guard<std::mutex> lk(srv->pmu);
+        srv->pending.insert(srv->pending.end(), b.begin(), b.end());
+    }
+    srv->pcv.notify_one();
+    return FIP_OK;
+}
+
+// After the last snapshot of a stream's launch sequence has completed: the
+// queued snapshots of that stream, if any, go out as the next sequence.  If
+// that launch fails, the error is kept in srv->launch_error (reported by
+// drain and by later submits) and the rest of the queue is released unlaunched.
+void relaunch(fip_server* srv, int sid)
+{
+    SubStream& ss = *srv->streams[sid];
+    Context* C = nullptr;
+    if (int rc = Context::get(&C)) {
+        std::lock_guard<std::mutex> lq(srv->qmu);
+        ss.running = false;
+        fail_slots(srv, ss, ss.queue, rc);
+        ss.queue.clear();
+        return;
+    }
+    std::lock_guard<std::recursive_mutex> lk(C->mu);
+    std::lock_guard<std::mutex> lq(srv->qmu);
+    ss.running = false;
+    if (int rc = launch_queued(srv, ss)) {
+        fail_slots(srv, ss, ss.queue, rc);
+        ss.queue.clear();
+    }
+}
+
 // Completion thread: polls the events of the snapshots in flight (any
 // stream, any order) and completes each as soon as its event has fired.
 void poll_loop(fip_server* srv)
@@ -222,7 +445,11 @@ void poll_loop(fip_server* srv)
                 std::lock_guard<std::mutex> lk(srv->pmu);
                 srv->pending.erase(std::find(srv->pending.begin(), srv->pending.end(), sl));
             }
+            // read before the slot returns to the idle list
+            const bool last = sl->last;
+            const int sid = sl->stream_id;
             on_complete(sl, t);
+            if (last) relaunch(srv, sid);
             any = true;
         }
         if (!any) std::this_thread::sleep_for(std::chrono::microseconds(5));
@@ -290,17 +517,19 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
             srv->streams.push_back(std::move(ss));
             e = hipStreamCreateWithFlags(&ssp->hs, hipStreamNonBlocking);
             if (e != hipSuccess) return fail(hip_error(e, "hipStreamCreate"));
-            const size_t G = ssp->G, M = ssp->M;
-            for (auto pr : {std::make_pair(&ssp->rec, N * sizeof(VisRec)), std::make_pair(&ssp->sorted, N * sizeof(VisRec)),
-                            std::make_pair(&ssp->bkslot, N * sizeof(int2)), std::make_pair(&ssp->hist, G * 4),
-                            std::make_pair(&ssp->bstart, (G + 1) * 4), std::make_pair(&ssp->kt, I * (M + 1) * 8),
-                            std::make_pair(&ssp->pre, I * I * 4), std::make_pair(&ssp->work, (size_t)256),
+            const size_t G = ssp->G, M = ssp->M, B = kServeBatch;
+            for (auto pr : {std::make_pair(&ssp->rec, B * N * sizeof(VisRec)),
+                            std::make_pair(&ssp->sorted, B * N * sizeof(VisRec)),
+                            std::make_pair(&ssp->bkslot, B * N * sizeof(int2)), std::make_pair(&ssp->hist, B * G * 4),
+                            std::make_pair(&ssp->bstart, B * (G + 1) * 4), std::make_pair(&ssp->kt, B * I * (M + 1) * 8),
+                            std::make_pair(&ssp->pre, B * I * I * 4), std::make_pair(&ssp->work, (size_t)256),
+                            std::make_pair(&ssp->batch_in, batch_layout(ssp->cfg).bytes),
                             std::make_pair(&ssp->img, (size_t)ring * I * I * 4),
                             std::make_pair(&ssp->meta, (size_t)ring * sizeof(SnapMeta)),
                             std::make_pair(&ssp->geom, (size_t)ring * sizeof(SnapGeom)),
                             std::make_pair(&ssp->pmax, pmax_words(ring) * sizeof(unsigned)),
                             std::make_pair(&ssp->win, (size_t)3 * sizeof(int) * ring),
-                            std::make_pair(&ssp->out, un * un * 4)}) {
+                            std::make_pair(&ssp->out, B * un * un * 4)}) {
                 e = pr.first->ensure(pr.second);
                 if (e != hipSuccess) return fail(hip_error(e, "hipMalloc (server)"));
             }
@@ -359,7 +588,7 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
         return set_error(FIP_ERR_BUDGET, "input pool of the HBM budget is full: snapshot refused");
     }
     sl->stream_id = sid;
-    bool enqueued = false;   // work reading the reservation may be in flight
+    bool enqueued = false;   // a copy reading the reservation may be in flight
     auto undo = [&](int rc) {
         if (enqueued) (void)hipStreamSynchronize(ss.hs);
         std::lock_guard<std::mutex> lk(srv->amu);
@@ -367,12 +596,9 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
         srv->idle.push_back(sl);
         return rc;
     };
-    Context* C = nullptr;
-    if (int rc = Context::get(&C)) return undo(rc);
-    std::lock_guard<std::recursive_mutex> lk(C->mu);
-    const size_t N = ss.cfg.num_baselines, I = ss.cfg.image_size;
+    // Host staging into the reservation's pinned mirror needs no lock.
+    const size_t N = ss.cfg.num_baselines;
     const size_t vb = N * esize(ss.cfg.vis_type);
-    const size_t un = I / ss.cfg.unit_size;
     const InLayout& L = ss.lay;
     char* h = static_cast<char*>(srv->hpool.p) + sl->off;
     char* d = srv->pool.as<char>() + sl->off;
@@ -381,98 +607,52 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     std::memcpy(h + L.b, bin, N * 8);
     std::memcpy(h + L.v, vin, 36);
     sl->arrival = arrival_s >= 0.0 ? arrival_s : t_in;
-    const long idx = ss.submitted;
-    const int slot = (int)(idx % srv->ring);
-    sl->window = idx >= 2 ? idx - 2 : -1;
-    hipStream_t st = ss.hs;
-#define SRV_HIP(x)                                              \
-    do {                                                        \
-        hipError_t e_ = (x);                                    \
-        if (e_ != hipSuccess) return undo(hip_error(e_, #x));   \
-    } while (0)
-    // one copy of the contiguous inputs (vis_re .. vin) of the reservation
-    enqueued = true;
-    SRV_HIP(hipMemcpyAsync(d, h, L.res, hipMemcpyHostToDevice, st));
-    WorkView w;
-    w.rec = ss.rec.as<VisRec>();
-    w.bkslot = ss.bkslot.as<int2>();
-    w.sorted = ss.sorted.as<VisRec>();
-    w.hist = ss.hist.as<int>();
-    w.bstart = ss.bstart.as<int>();
-    w.meta = ss.meta.as<SnapMeta>() + slot;
-    w.geom = ss.geom.as<SnapGeom>() + slot;
-    w.kt = ss.kt.as<float2>();
-    w.pre = ss.pre.as<float>();
-    w.img = ss.img.as<float>() + (size_t)slot * I * I;
-    w.work = ss.work.as<unsigned>();
-    w.pmax = ss.pmax.as<unsigned>() + pmax_index(slot, 0, 0);
-    w.stride = N;
-    InputView in{d, d + L.vi, reinterpret_cast<const float*>(d + L.b), reinterpret_cast<const float*>(d + L.v),
-                 nullptr, N, (int)N, ss.cfg.vis_type};
-    const PlanTables& p = ss.plan->t;
-    const double scale = (double)ss.cfg.cell_size * (double)p.G;
-    SRV_HIP(launch_reset(in, w, p, 1, ss.cfg.cell_size, nullptr, (int)N, st));
-    SRV_HIP(launch_bin(in, w, p, 1, (int)N, scale, st));
-    SRV_HIP(launch_scan(w, p, 1, st));
-    SRV_HIP(launch_scatter(w, p, 1, (int)N, st));
-    SRV_HIP(launch_stage1(w, p, 1, ss.tu, st));
-    SRV_HIP(launch_stage2(w, p, 1, ss.tu, st));
-    const int T = (int)ss.cfg.unit_size;
-    if (sl->window >= 0) {
-        // The window (idx-2, idx-1, idx) in ring slots.
-        WorkView wr = w;
-        wr.meta = ss.meta.as<SnapMeta>();
-        wr.pmax = ss.pmax.as<unsigned>();
-        wr.img = ss.img.as<float>();
-        float* dout = ss.out.as<float>();
-        if (splat_trigger_fusable(p, ss.tu, T)) {
-            // This snapshot's splat carries the trigger; its image is kept for
-            // the next two windows.
-            const int sa = (int)((idx - 2) % srv->ring), sb = (int)((idx - 1) % srv->ring);
-            SRV_HIP(launch_splat_trigger(w, p, 0, wr.img + (size_t)sa * I * I, wr.img + (size_t)sb * I * I, true,
-                                         st));
-            SRV_HIP(launch_trigger_finish(wr, p, sa, sb, slot, T, dout, st));
-        } else {
-            SRV_HIP(launch_splat(w, p, 1, ss.tu, st));
-            SRV_HIP(launch_trigger(wr, p, ss.win.as<int>() + 3 * slot, 1, T, dout, st));
-        }
-        SRV_HIP(hipMemcpyAsync(h + L.res, dout, un * un * 4, hipMemcpyDeviceToHost, st));
-    } else {
-        SRV_HIP(launch_splat(w, p, 1, ss.tu, st));
+    Context* C = nullptr;
+    if (int rc = Context::get(&C)) return undo(rc);
+    std::lock_guard<std::recursive_mutex> lk(C->mu);
+    {
+        std::lock_guard<std::mutex> lk2(srv->mu);
+        if (srv->launch_error) return undo(srv->launch_error);
     }
+    const long idx = ss.submitted;
+    sl->index = idx;
+    sl->ring_slot = (int)(idx % srv->ring);
+    sl->window = idx >= 2 ? idx - 2 : -1;
+    // one copy of the contiguous inputs (vis_re .. vin) of the reservation,
+    // ordered before any launch sequence that reads it
+    enqueued = true;
+    {
+        const hipError_t e = hipMemcpyAsync(d, h, L.res, hipMemcpyHostToDevice, ss.hs);
+        if (e != hipSuccess) return undo(hip_error(e, "hipMemcpyAsync (server input)"));
+    }
+    ss.submitted += 1;
     {
         std::lock_guard<std::mutex> lk2(srv->mu);
         srv->inflight += 1;
     }
-    hipError_t e = hipEventRecord(sl->done, st);
-    if (e != hipSuccess) {
-        std::lock_guard<std::mutex> lk2(srv->mu);
-        srv->inflight -= 1;
-        return undo(hip_error(e, "hipEventRecord"));
-    }
-    {
-        std::lock_guard<std::mutex> lk2(srv->pmu);
-        srv->pending.push_back(sl);
-    }
-    srv->pcv.notify_one();
-#undef SRV_HIP
-    ss.submitted += 1;
-    return FIP_OK;
+    std::lock_guard<std::mutex> lq(srv->qmu);
+    ss.queue.push_back(sl);
+    if (ss.running) return FIP_OK;   // the completion thread launches it with the next sequence
+    return launch_queued(srv, ss);
 }
 
 int fip_server_drain(fip_server* srv)
 {
     if (!srv) return set_error(FIP_ERR_ARG, "null server");
-    for (auto& ss : srv->streams) FIP_HIP(hipStreamSynchronize(ss->hs));
-    // Every event has fired; wait for the completion thread to record them.
-    for (int spin = 0; spin < 200000; ++spin) {
+    // Queued snapshots are launched by the completion thread when the
+    // sequence ahead of them completes, so the streams are synchronised
+    // again until the completion thread has recorded every snapshot.
+    const double deadline = now_s() + 120.0;
+    for (;;) {
+        for (auto& ss : srv->streams) FIP_HIP(hipStreamSynchronize(ss->hs));
         {
             std::lock_guard<std::mutex> lk(srv->mu);
-            if (srv->inflight <= 0) return FIP_OK;
+            if (srv->inflight <= 0) return srv->launch_error ? set_error(srv->launch_error, "a queued launch failed")
+                                                             : FIP_OK;
         }
+        if (now_s() > deadline) return set_error(FIP_ERR_HIP, "completions did not arrive");
         std::this_thread::sleep_for(std::chrono::microseconds(10));
     }
-    return set_error(FIP_ERR_HIP, "completions did not arrive");
 }
 
 int fip_server_stats(fip_server* srv, double* p50_ms, double* p99_ms, double* mean_ms, long* processed, long* refused)
@@ -503,6 +683,16 @@ int fip_server_stats(fip_server* srv, double* p50_ms, double* p99_ms, double* me
     return FIP_OK;
 }
 
+int fip_server_sequences(fip_server* srv, long* sequences)
+{
+    if (!srv || !sequences) return set_error(FIP_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lq(srv->qmu);
+    long n = 0;
+    for (auto& ss : srv->streams) n += ss->sequences;
+    *sequences = n;
+    return FIP_OK;
+}
+
 int fip_server_latest(fip_server* srv, int sid, float* tlisi, long* window)
 {
     if (!srv || !tlisi) return set_error(FIP_ERR_ARG, "null argument");
@@ -527,7 +717,7 @@ void fip_server_destroy(fip_server* srv)
     if (srv->poller.joinable()) srv->poller.join();
     for (auto& ss : srv->streams) {
         for (DevBuf* b : {&ss->rec, &ss->bkslot, &ss->sorted, &ss->hist, &ss->bstart, &ss->kt, &ss->pre, &ss->work,
-                          &ss->img, &ss->meta, &ss->geom, &ss->pmax, &ss->win, &ss->out})
+                          &ss->batch_in, &ss->img, &ss->meta, &ss->geom, &ss->pmax, &ss->win, &ss->out})
             b->release();
         if (ss->hs) (void)hipStreamDestroy(ss->hs);
     }

@@ -289,6 +289,11 @@ int fip_server_drain(fip_server *srv);
 /* Latency statistics (ms, arrival -> tLISI on host) and counts. */
 int fip_server_stats(fip_server *srv, double *p50_ms, double *p99_ms,
                      double *mean_ms, long *processed, long *refused);
+/* Launch sequences enqueued so far, summed over the streams.  A snapshot
+ * submitted while its stream's previous sequence is still running waits and
+ * is launched with the others queued behind it (up to 3 per sequence, fewer
+ * where the ring wraps), so processed / sequences > 1 under load. */
+int fip_server_sequences(fip_server *srv, long *sequences);
 /* Latest tLISI matrix of a stream ((image/tile)^2 floats) and its window
  * index; FIP_ERR_ARG if the stream has fewer than 3 images yet. */
 int fip_server_latest(fip_server *srv, int sid, float *tlisi, long *window);

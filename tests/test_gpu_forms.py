@@ -8,7 +8,7 @@ import time
 import numpy as np
 import pytest
 
-from tests.inputs import (LARGE, SMALL, from_bf16_bits, rel_rms, synth_workload, to_bf16_bits,
+from tests.inputs import (LARGE, SMALL, from_bf16_bits, rel_rms, snapshots, synth_workload, to_bf16_bits,
                           triggered)
 
 pytestmark = pytest.mark.gpu
@@ -302,6 +302,45 @@ def test_serving_streams_match_pipe(fip):
         assert np.abs(t1 - e1).max() < 1e-6
     finally:
         srv.close()
+
+
+def test_serving_batches_queued_snapshots(fip):
+    """Snapshots submitted back to back queue behind their stream's running
+    launch sequence and go out together, up to three per sequence in
+    consecutive ring slots, never over the two images that the sequence's
+    first window reads.  For every count from 3 to 8 snapshots on a ring of 5
+    (sequences start at every slot and the ring wraps) the latest window must
+    be the same bits as FIpipe of the last three snapshots, at a 32-pixel tile
+    (a one-snapshot sequence carries the trigger in its splat) and at 16
+    (separate trigger), and some sequences must have held several snapshots."""
+    image, nbl, cell = 1024, 8000, 1.45e-5
+    vr, vi, B, V = snapshots(31, 8, nbl, image, cell, transient_snap=4)
+    total = seqs = 0
+    for tile in (32, 16):
+        cfg = [dict(num_baselines=nbl, image_size=image, cell_size=cell, unit_size=tile)]
+        for n in range(3, 9):
+            srv = fip.Server(cfg, ring=5, hbm_budget_bytes=2 << 30)
+            try:
+                for s in range(n):
+                    # the input pool holds one reservation per ring slot:
+                    # a refused arrival is offered again once one frees
+                    while not srv.submit(0, vr[s], vi[s], B[s], V[s]):
+                        time.sleep(1e-4)
+                srv.drain()
+                st = srv.stats()
+                got, win = srv.latest(0)
+                nseq = srv.sequences()
+            finally:
+                srv.close()
+            ref = fip.fipipe(vr[n - 3:n], vi[n - 3:n], B[n - 3:n], V[n - 3:n], nbl, image, 3, cell, tile)
+            print(f"tile {tile}, {n} snapshots: {nseq} sequences, window {win}, "
+                  f"max |serving - FIpipe| {np.abs(got - ref).max():.3g}")
+            assert st["processed"] == n and win == n - 3
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+            assert 1 <= nseq <= n
+            total += n
+            seqs += nseq
+    assert seqs < total
 
 
 def test_serving_admission_per_arrival(fip):
