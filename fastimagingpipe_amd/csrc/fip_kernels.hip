@@ -783,6 +783,79 @@ struct C2RGather {
     }
 };
 
+// The same gather for a K band narrower than half the row (cmax < M/2), the
+// usual case (the uv coverage spans under half the Nyquist band): every
+// K[M-p] of a pair is then zero, so only K[p] is read and the pairing keeps
+// the operations on the non-zero terms, Z[p] = K[p] + i K[p] w_p and
+// Z[M-p] = conj(K[p]) + i (-conj(K[p])) w_{M-p}.  Pair blocks that start
+// beyond cmax only write zeros.
+template <int M, int ROWS>
+struct C2RGatherLow {
+    static constexpr int NT = FftPlan<M>::nt * ROWS;
+    static constexpr int PB = (M / 2 + NT) / NT;
+    float2 a[PB][ROWS], wv[PB];
+
+    __device__ __forceinline__ void load(const float2* __restrict__ kt0, const float2* __restrict__ wpost,
+                                         int cmax, int tid)
+    {
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            if (u * NT > cmax) break;
+            const int p = tid + u * NT;
+            const bool ok = p <= cmax;
+            wv[u] = ok ? wpost[p] : make_float2(0.0f, 0.0f);
+            if constexpr (ROWS % 2 == 0) {
+#pragma unroll
+                for (int r = 0; r < ROWS; r += 2) {
+                    const float4 va = ok ? *reinterpret_cast<const float4*>(kt0 + (size_t)p * kKTile + r)
+                                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    a[u][r] = make_float2(va.x, va.y);
+                    a[u][r + 1] = make_float2(va.z, va.w);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r)
+                    a[u][r] = ok ? kt0[(size_t)p * kKTile + r] : make_float2(0.0f, 0.0f);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(float2* zb, int cmax, int tid) const
+    {
+        constexpr int L = lds_len(M + 1);
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            const int p = tid + u * NT;
+            if (p > M / 2) break;
+            if (u * NT > cmax) {
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r) {
+                    float2* zr = zb + r * L;
+                    zr[lds_pad(p)] = make_float2(0.0f, 0.0f);
+                    zr[lds_pad(M - p)] = make_float2(0.0f, 0.0f);
+                }
+                continue;
+            }
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+                float2* zr = zb + r * L;
+                const float2 av = a[u][r];
+                if (p == 0) {
+                    zr[lds_pad(0)] = cadd(av, cmul_i(av));
+                } else {
+                    zr[lds_pad(p)] = cadd(av, cmul_i(cmul(av, wv[u])));
+                    if (p != M - p) {
+                        const float2 km = cconj(av);
+                        const float2 nk = make_float2(-av.x, av.y);   // -conj(K[p])
+                        const float2 wm = make_float2(-wv[u].x, wv[u].y);
+                        zr[lds_pad(M - p)] = cadd(km, cmul_i(cmul(nk, wm)));
+                    }
+                }
+            }
+        }
+    }
+};
+
 // First K tile run of image row y0 of snapshot s.
 template <int M>
 __device__ __forceinline__ const float2* c2r_rows(const float2* __restrict__ kt, int s, int y0, int I)
@@ -840,6 +913,65 @@ __device__ __forceinline__ float c2r_output_rows(const float2* zb, const float* 
     return lmax;
 }
 
+// The image size of a plan: G = 2M = 1.5 I.
+template <int M>
+constexpr int c2r_image() { return 4 * M / 3; }
+
+// Whether c2r_output_fixed applies: each of a thread's columns lies wholly on
+// one side of the crop's wrap point h, and every column's offset from the
+// thread's first one is a multiple of 64.
+template <int M, int NTB>
+constexpr bool c2r_fixed_ok()
+{
+    constexpr int G = 2 * M, I = c2r_image<M>(), h = I / 2;
+    return 3 * I == 2 * G && NTB % 64 == 0 && h % 64 == 0 && (G - h) % 64 == 0 && NTB <= h && h % NTB == 0;
+}
+
+// c2r_output_rows for the plan's own image size, with the column walk resolved
+// at compile time: thread t takes columns t + j*NTB.  Column col maps to the
+// real sample mm = col + G - h (col < h) or col - h (col >= h), i.e. t plus a
+// constant C_j that is a multiple of 64, so its float in the LDS row,
+// 2*lds_pad(mm >> 1) + (mm & 1), is t + 2*(t >> 6) + C_j + C_j/32: one 4-byte
+// LDS read at an immediate offset from a per-thread base, in place of an
+// index computation, an 8-byte read and a select.  The same values, products
+// and maxima as c2r_output_rows.
+template <int M, int ROWS, int NTB>
+__device__ __forceinline__ float c2r_output_fixed(const float2* zb, const float* __restrict__ corr, float* out0,
+                                                  int y0, int tid)
+{
+    constexpr int G = 2 * M, I = c2r_image<M>(), h = I / 2;
+    constexpr int L2 = 2 * lds_len(M + 1);   // floats per LDS row
+    constexpr int NC = I / NTB;              // columns per thread
+    float lmax = 0.0f;
+    float cy[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int y = y0 + r - h;
+        cy[r] = corr[y < 0 ? -y : y];
+    }
+    const float* zf = reinterpret_cast<const float*>(zb) + tid + 2 * (tid >> 6);
+    float cv[NC];
+    static_for<NC>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if constexpr ((j + 1) * NTB <= h)
+            cv[j] = corr[(h - j * NTB) - tid];   // |x| = h - col
+        else
+            cv[j] = corr[tid + (j * NTB - h)];   // |x| = col - h
+    });
+    static_for<NC>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        constexpr int C = ((j + 1) * NTB <= h) ? j * NTB + (G - h) : j * NTB - h;
+        constexpr int off = C + C / 32;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const float val = fabsf(zf[r * L2 + off]) * (cv[j] * cy[r]);
+            __builtin_nontemporal_store(val, out0 + (size_t)r * I + tid + j * NTB);
+            lmax = fmaxf(lmax, val);
+        }
+    });
+    return lmax;
+}
+
 template <int M, int ROWS>
 __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* __restrict__ kt,
                                                                    const SnapMeta* __restrict__ meta,
@@ -857,7 +989,11 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
     const int y0 = xcd_remap(blockIdx.x, gridDim.x) * ROWS;
     const int grp = threadIdx.x / NT, t = threadIdx.x % NT;
     const int cmax = min(meta[s].rmax, M);
-    {
+    if (cmax < M / 2) {
+        C2RGatherLow<M, ROWS> gth;
+        gth.load(c2r_rows<M>(kt, s, y0, I), wpost, cmax, threadIdx.x);
+        gth.store(zb, cmax, threadIdx.x);
+    } else {
         C2RGather<M, ROWS> gth;
         gth.load(c2r_rows<M>(kt, s, y0, I), wpost, cmax, threadIdx.x);
         gth.store(zb, threadIdx.x);
@@ -865,8 +1001,14 @@ __global__ __launch_bounds__(FftPlan<M>::nt * ROWS) void k_stage2(const float2* 
     __syncthreads();
     float2* z = zb + grp * L;
     fft_inverse<M>(z, tw, t);
-    const float lmax = c2r_output_rows<M, ROWS, NT * ROWS>(zb, corr, pre + (size_t)s * I * I + (size_t)y0 * I, y0,
-                                                          I, threadIdx.x);
+    float* const out0 = pre + (size_t)s * I * I + (size_t)y0 * I;
+    float lmax;
+    if constexpr (c2r_fixed_ok<M, NT * ROWS>()) {
+        lmax = I == c2r_image<M>() ? c2r_output_fixed<M, ROWS, NT * ROWS>(zb, corr, out0, y0, threadIdx.x)
+                                   : c2r_output_rows<M, ROWS, NT * ROWS>(zb, corr, out0, y0, I, threadIdx.x);
+    } else {
+        lmax = c2r_output_rows<M, ROWS, NT * ROWS>(zb, corr, out0, y0, I, threadIdx.x);
+    }
     const unsigned bm = block_max_u<NT * ROWS>(__float_as_uint(lmax), shm);
     if (threadIdx.x == 0) atomicMax(max_slot(pmax, s, 0, (int)(blockIdx.x % kMaxSlots)), bm);
 }
