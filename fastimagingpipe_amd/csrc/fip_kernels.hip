@@ -244,6 +244,8 @@ __global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ n
 // CUs busy although a step has only ~100 workgroups.
 constexpr int kBinThreads = 1024;
 constexpr int kBinVpt = 4;
+// Largest grid size: bounds the bucket counts one bin or scan thread holds.
+constexpr int kMaxGrid = 12288;
 
 // Block-privatised column histogram: slots within the block come from LDS
 // atomics, then one global atomic per non-empty bucket reserves the block's
@@ -269,6 +271,25 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(InputView in, VisRec* __res
     const float* V = in.vin + (size_t)s * 9;
     // computeVisWeighted: v / |V00 V11 - V01 V10| (FIPkernels.cu:188-195)
     const double J = fabs((double)V[0] * (double)V[4] - (double)V[1] * (double)V[3]);
+    // Every input of the thread's visibilities is loaded before any is used,
+    // so the loads are in flight together: unconditionally, from an index
+    // clamped into this snapshot's rows (entries past its count are loaded
+    // but not used), since a load behind a per-lane test would wait in turn.
+    float lvr[kBinVpt], lvi[kBinVpt], lb1[kBinVpt], lb2[kBinVpt];
+    const int nlast = max(ns - 1, 0);
+#pragma unroll
+    for (int k = 0; k < kBinVpt; ++k) {
+        const int n = min(blockIdx.x * (kBinThreads * kBinVpt) + k * kBinThreads + threadIdx.x, nlast);
+        const size_t q = (size_t)s * in.stride + (size_t)n;
+        lvr[k] = vis_to_f<VT>(in.vis_re, q);
+        lvi[k] = vis_to_f<VT>(in.vis_im, q);
+        lb1[k] = in.bin[2 * q];
+        lb2[k] = in.bin[2 * q + 1];
+    }
+    // Keeps the loads above the per-visibility tests (the compiler would
+    // otherwise sink each into the branch that uses it).
+#pragma unroll
+    for (int k = 0; k < kBinVpt; ++k) asm volatile("" ::"v"(lvr[k]), "v"(lvi[k]), "v"(lb1[k]), "v"(lb2[k]));
 #pragma unroll
     for (int k = 0; k < kBinVpt; ++k) {
         bk[k] = -1;
@@ -276,11 +297,11 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(InputView in, VisRec* __res
         const int n = blockIdx.x * (kBinThreads * kBinVpt) + k * kBinThreads + threadIdx.x;
         if (n >= ns) continue;
         const size_t q = (size_t)s * in.stride + (size_t)n;
-        const float vr = (float)((double)vis_to_f<VT>(in.vis_re, q) / J);
-        const float vi = (float)((double)vis_to_f<VT>(in.vis_im, q) / J);
+        const float vr = (float)((double)lvr[k] / J);
+        const float vi = (float)((double)lvi[k] / J);
         // grid coordinates in FP64: B * (cell * g) (FIPkernels.cu:211-212, 584)
-        const double p1 = (double)in.bin[2 * q] * scale;
-        const double p2 = (double)in.bin[2 * q + 1] * scale;
+        const double p1 = (double)lb1[k] * scale;
+        const double p2 = (double)lb2[k] * scale;
         // Coordinates far outside the grid have no taps (and must not overflow int).
         if (!(fabs(p1) < (double)G && fabs(p2) < (double)G)) continue;
         const double fb1 = floor(p1), fb2 = floor(p2);
@@ -303,9 +324,26 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(InputView in, VisRec* __res
         vmax = max(vmax, __float_as_uint(fmaxf(fabsf(vr), fabsf(vi))));
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < G; b += kBinThreads) {
-        const int c = lh[b];
-        if (c > 0) lh[b] = atomicAdd(&hist[(size_t)s * G + b], c);
+    // Flush: each thread's bucket counts are read, then all of its global
+    // atomics are issued before the first result is used.
+    {
+        constexpr int FL = (kMaxGrid + kBinThreads - 1) / kBinThreads;
+        int fc[FL];
+#pragma unroll
+        for (int k = 0; k < FL; ++k) {
+            const int b = threadIdx.x + k * kBinThreads;
+            fc[k] = b < G ? lh[b] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < FL; ++k) {
+            const int b = threadIdx.x + k * kBinThreads;
+            if (fc[k] > 0) fc[k] = atomicAdd(&hist[(size_t)s * G + b], fc[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < FL; ++k) {
+            const int b = threadIdx.x + k * kBinThreads;
+            if (b < G) lh[b] = fc[k];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -344,9 +382,6 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(InputView in, VisRec* __res
 }
 
 // ------------------------------------------------------------------ scan
-
-// Largest grid size: bounds the bucket counts one scan thread holds.
-constexpr int kMaxGrid = 12288;
 
 __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ hist, int* __restrict__ bstart, int G)
 {
