@@ -520,10 +520,18 @@ __device__ __forceinline__ void grid_taps(const ColRec& c, int M, int G, int rbs
     float x = ((float)(lo1 - c.b1) - c.f1) * 0.25f;
     const int sgn = c.mirror ? -1 : 1;
     int row = sgn * lo1;   // the tap row sgn * t1, stepped with t1
+    // The taps' rows are one contiguous run of r mod G, so they reach at most
+    // two scale blocks: the first tap's and the last tap's.  Both scales are
+    // read once; each tap selects its own, so no tap waits on an LDS read
+    // (which would also wait for the previous taps' atomics).
+    const int ra = row, rz = sgn * hi1;
+    const unsigned ba = (unsigned)(ra < 0 ? ra + G : ra) / (unsigned)rbs;
+    const unsigned bz = (unsigned)(rz < 0 ? rz + G : rz) / (unsigned)rbs;
+    const float sca = sc[ba], scz = sc[bz];
     auto tap = [&]() {
         const int idx = row < 0 ? row + G : row;
         const int q = 2 * lds_pad(idx);
-        const float wk = es_in(x) * sc[(unsigned)idx / (unsigned)rbs];
+        const float wk = es_in(x) * ((unsigned)idx / (unsigned)rbs == ba ? sca : scz);
         atomicAdd(&col[q], cvt_round_half_up_i32(c.vr * wk));
         atomicAdd(&col[q + 1], cvt_round_half_up_i32(c.vi * wk));
         x += 0.25f;
