@@ -179,3 +179,27 @@ def test_raw_chain_argument_errors_before_any_device_use(F):
     _expect(F, -3, F.pipe_raw_device, ptrs, 2016, 1, 3, 100, 5.8e-5, 16, fake)     # image size
     _expect(F, -3, F.pipe_raw_device, ptrs, 1 << 30, 1 << 20, 3, 256, 5.8e-5, 16, fake)   # nrows*nchan
     _expect(F, -1, F.pipe_raw_device, dict(ptrs, w3=0), 2016, 1, 3, 256, 5.8e-5, 16, fake)
+
+
+def test_server_argument_errors_before_any_device_use(F):
+    """The serving entry points reject null and malformed arguments before they
+    create a context or touch a device."""
+    lib = F.lib
+    n = ctypes.c_long(7)
+    assert lib.fip_server_sequences(None, ctypes.byref(n)) == -1
+    assert n.value == 7
+    assert lib.fip_server_drain(None) == -1
+    assert lib.fip_server_stats(None, None, None, None, None, None) == -1
+    out = ctypes.c_void_p()
+    cfg = (F._StreamConfig * 1)()
+    cfg[0].num_baselines, cfg[0].image_size, cfg[0].cell_size, cfg[0].unit_size = 2016, 256, 5.8e-5, 16
+    assert lib.fip_server_create(cfg, 1, 2, 1 << 30, ctypes.byref(out)) == -1     # ring < 3
+    assert lib.fip_server_create(cfg, 0, 8, 1 << 30, ctypes.byref(out)) == -1     # no stream
+    cfg[0].vis_type = 9
+    assert lib.fip_server_create(cfg, 1, 8, 1 << 30, ctypes.byref(out)) == -1     # bad vis_type
+    cfg[0].vis_type = 0
+    cfg[0].image_size = 100
+    assert lib.fip_server_create(cfg, 1, 8, 1 << 30, ctypes.byref(out)) != 0      # unsupported size
+    cfg[0].image_size = 256
+    assert lib.fip_server_create(cfg, 1, 8, 1 << 16, ctypes.byref(out)) == -7     # budget, before any device use
+    assert not out.value
