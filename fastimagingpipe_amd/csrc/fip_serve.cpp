@@ -15,8 +15,10 @@
 // sequence is still running waits in the stream's queue; when that sequence
 // completes, the completion thread launches the queued snapshots (up to
 // kServeBatch, in consecutive ring slots) as one multi-snapshot sequence.
-// An idle stream launches a snapshot at once, alone, so the latency at low
-// load is that of one snapshot.
+// Once a full batch has queued up, it is enqueued at once behind the running
+// sequence (at most two per stream), so a saturated stream keeps its GPU
+// queue filled.  An idle stream launches a snapshot at once, alone, so the
+// latency at low load is that of one snapshot.
 //
 // Admission by HBM budget: at creation every stream's fixed device state
 // (work buffers, ring, window table) plus one snapshot of the largest
@@ -38,6 +40,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <sys/prctl.h>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -158,10 +161,11 @@ struct SubStream {
     DevBuf img, meta, geom, pmax;                              // ring
     DevBuf win, out;
     InLayout lay{};
-    // Admitted snapshots waiting for the running launch sequence (guarded by
-    // the server's qmu), and whether one is running.
+    // Admitted snapshots waiting for a launch sequence, and the sequences
+    // enqueued and not yet completed (at most kMaxRunning); both guarded by
+    // the server's qmu.
     std::vector<InSlot*> queue;
-    bool running = false;
+    int running = 0;
     long sequences = 0;   // launch sequences enqueued (guarded by qmu)
     long submitted = 0;
     std::vector<float> latest;
@@ -233,7 +237,7 @@ struct fip_server {
     std::vector<InSlot*> pending;  // launched, event not yet seen complete
     bool stop = false;
     std::thread poller;
-    std::mutex qmu;                // guards the streams' queues and running flags
+    std::mutex qmu;                // guards the streams' queues and running counts
     int launch_error = 0;          // first failed launch of the completion thread (under mu)
 };
 
@@ -285,7 +289,7 @@ void fail_slots(fip_server* srv, SubStream& ss, const std::vector<InSlot*>& b, i
 // the batch layout (one snapshot is read from its reservation in place), the
 // imaging, the windows they complete, the tLISI copies and one event per
 // snapshot, which the completion thread then watches.  The caller holds the
-// context lock and srv->qmu, and ss has no sequence running.
+// context lock and srv->qmu.
 //
 // A sequence images all of its snapshots before its trigger reads the first
 // window, whose two earlier images sit in ring slots r0-2 and r0-1 (mod ring)
@@ -389,7 +393,7 @@ int launch_queued(fip_server* srv, SubStream& ss)
         SRV_HIP(hipEventRecord(b[j]->done, st));
     }
 #undef SRV_HIP
-    ss.running = true;
+    ss.running += 1;
     ss.sequences += 1;
     {
         std::lock_guard<std::mutex> lk(srv->pmu);
@@ -399,34 +403,62 @@ int launch_queued(fip_server* srv, SubStream& ss)
     return FIP_OK;
 }
 
+// Launch sequences per stream enqueued at once: the running one and the next.
+constexpr int kMaxRunning = 2;
+
+// Largest sequence that launch_queued would take from the front of the queue.
+int batch_limit(const fip_server* srv, const SubStream& ss)
+{
+    const int ring = srv->ring, r0 = ss.queue.front()->ring_slot;
+    return std::min({kServeBatch, ring - r0, ring - 2});
+}
+
+// Launches queued snapshots of ss while that is useful: any queue when no
+// sequence is running, a full batch behind a running one.  The caller holds
+// the context lock and srv->qmu.  A failed launch releases the rest of the
+// queue unlaunched and returns its error (also kept in srv->launch_error).
+int launch_ready(fip_server* srv, SubStream& ss)
+{
+    while (!ss.queue.empty() && ss.running < kMaxRunning &&
+           (ss.running == 0 || (int)ss.queue.size() >= batch_limit(srv, ss))) {
+        if (int rc = launch_queued(srv, ss)) {
+            fail_slots(srv, ss, ss.queue, rc);
+            ss.queue.clear();
+            return rc;
+        }
+    }
+    return FIP_OK;
+}
+
 // After the last snapshot of a stream's launch sequence has completed: the
-// queued snapshots of that stream, if any, go out as the next sequence.  If
-// that launch fails, the error is kept in srv->launch_error (reported by
-// drain and by later submits) and the rest of the queue is released unlaunched.
+// queued snapshots of that stream, if any, go out.  If that launch fails, the
+// error is kept in srv->launch_error (reported by drain and by later
+// submits) and the rest of the queue is released unlaunched.
 void relaunch(fip_server* srv, int sid)
 {
     SubStream& ss = *srv->streams[sid];
     Context* C = nullptr;
     if (int rc = Context::get(&C)) {
         std::lock_guard<std::mutex> lq(srv->qmu);
-        ss.running = false;
+        ss.running -= 1;
         fail_slots(srv, ss, ss.queue, rc);
         ss.queue.clear();
         return;
     }
     std::lock_guard<std::recursive_mutex> lk(C->mu);
     std::lock_guard<std::mutex> lq(srv->qmu);
-    ss.running = false;
-    if (int rc = launch_queued(srv, ss)) {
-        fail_slots(srv, ss, ss.queue, rc);
-        ss.queue.clear();
-    }
+    ss.running -= 1;
+    (void)launch_ready(srv, ss);
 }
 
 // Completion thread: polls the events of the snapshots in flight (any
 // stream, any order) and completes each as soon as its event has fired.
 void poll_loop(fip_server* srv)
 {
+    // The short sleeps between polls are meant literally: with the default
+    // timer slack (50 us) each would last ~50 us, and a stream's next launch
+    // sequence would wait that long after the previous one completes.
+    (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
     std::vector<InSlot*> work;
     for (;;) {
         {
@@ -632,8 +664,8 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     }
     std::lock_guard<std::mutex> lq(srv->qmu);
     ss.queue.push_back(sl);
-    if (ss.running) return FIP_OK;   // the completion thread launches it with the next sequence
-    return launch_queued(srv, ss);
+    // launched now, or by the completion thread with the next sequence
+    return launch_ready(srv, ss);
 }
 
 int fip_server_drain(fip_server* srv)
