@@ -206,10 +206,20 @@ def bench_serving(F, seconds=2.0, rate=200.0, saturate=120):
                    "mean_ms": round(st["mean_ms"], 4), "sustained_per_s": round(st["processed"] / elapsed, 1)}
     finally:
         srv.close()
-    # Saturation: the 4096^2 stream alone, submitted back to back.
+    # Saturation: the 4096^2 stream alone, submitted back to back, after an
+    # untimed pass on a server of its own (first launches of the kernels that
+    # only batched sequences use, first use of a server's buffers).
+    vr, vi, b, v = data[1]
+    warm = F.Server([streams[1]], ring=8, hbm_budget_bytes=48 << 30)
+    try:
+        for k in range(24):
+            while not warm.submit(0, vr[k % 4], vi[k % 4], b[k % 4], v[k % 4], -1.0):
+                pass
+        warm.drain()
+    finally:
+        warm.close()
     srv = F.Server([streams[1]], ring=8, hbm_budget_bytes=48 << 30)
     try:
-        vr, vi, b, v = data[1]
         t0 = time.monotonic()
         done = 0
         in_submit = 0.0   # host time inside accepted submits (staging copy included)

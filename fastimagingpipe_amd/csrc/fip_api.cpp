@@ -109,7 +109,7 @@ int pipe_host(const void* vis_re, const void* vis_im, int vt, const float* bin, 
             InputView in{d, d + o_vi, reinterpret_cast<const float*>(d + o_b), reinterpret_cast<const float*>(d + o_v),
                          nullptr, nbl, (int)nbl, vt};
             if (int rc = imaging_sequence(C, P->t, snapshot_view(wall, s, P->t), in, 1, (int)nbl, cell, nullptr,
-                                          (int)nbl, tu, st, fuse && s == S - 1))
+                                          (int)nbl, tu, st, fuse && s == S - 1 ? 1 : 0))
                 return rc;
         }
         if (fuse) {

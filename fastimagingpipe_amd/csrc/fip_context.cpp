@@ -343,7 +343,7 @@ int Context::collect_stages()
 
 int imaging_sequence(Context* C, const PlanTables& p, const WorkView& w, const InputView& in, int S, int nmax,
                      float cell, const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st,
-                     bool defer_last)
+                     int ndefer)
 {
     // r1r2_scale = cell * g (FIPkernels.cu:584), kept in FP64.
     const double scale = (double)cell * (double)p.G;
@@ -353,7 +353,7 @@ int imaging_sequence(Context* C, const PlanTables& p, const WorkView& w, const I
     FIP_STAGE(C, kStScatter, st, launch_scatter(w, p, S, nmax, st));
     FIP_STAGE(C, kStGridFft, st, launch_stage1(w, p, S, tu, st));
     FIP_STAGE(C, kStRowC2R, st, launch_stage2(w, p, S, tu, st));
-    const int ns = defer_last ? S - 1 : S;
+    const int ns = S - ndefer;
     if (ns > 0) FIP_STAGE(C, kStSplat, st, launch_splat(w, p, ns, tu, st));
     return FIP_OK;
 }
@@ -378,7 +378,7 @@ int pipe_sequence(Context* C, const PlanTables& p, const WorkView& w, const Inpu
                   const Tuning& tu, const int* win_dev, int tile, float* out_dev, hipStream_t st)
 {
     const bool fuse = splat_trigger_fusable(p, tu, tile);
-    if (int rc = imaging_sequence(C, p, w, in, 3, nmax, cell, nullptr, nmax, tu, st, fuse)) return rc;
+    if (int rc = imaging_sequence(C, p, w, in, 3, nmax, cell, nullptr, nmax, tu, st, fuse ? 1 : 0)) return rc;
     if (fuse) return fused_trigger_sequence(C, p, w, tile, out_dev, st);
     return trigger_sequence(C, p, w, win_dev, 1, tile, out_dev, st);
 }

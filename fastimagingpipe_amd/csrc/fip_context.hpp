@@ -132,11 +132,12 @@ private:
 WorkView snapshot_view(const WorkView& w, int s, const PlanTables& p);
 
 // Enqueue reset..splat / the trigger on explicit tables and buffers; C (may
-// be null) only adds the profiling events.  With defer_last the splat of the
-// view's last snapshot is left to fused_trigger_sequence.
+// be null) only adds the profiling events.  The splats of the view's last
+// ndefer snapshots are left to the caller's fused trigger launches
+// (fused_trigger_sequence for FIpipe's third snapshot).
 int imaging_sequence(Context* C, const PlanTables& p, const WorkView& w, const InputView& in, int S, int nmax,
                      float cell, const int* norm_dev, int norm_const, const Tuning& tu, hipStream_t st,
-                     bool defer_last = false);
+                     int ndefer = 0);
 int trigger_sequence(Context* C, const PlanTables& p, const WorkView& w, const int* win_dev, int nwin, int tile,
                      float* out_dev, hipStream_t st);
 // The fused tail of window (0, 1, 2) of a three-snapshot view whose last

@@ -1,24 +1,26 @@
 // Serving form: streaming imaging + trigger for several sub-array streams.
 //
-// Each stream owns a HIP stream (default priority), work buffers for
-// kServeBatch snapshots, a device ring of `ring` image slots and its window
-// table.  A submitted snapshot reserves room for its inputs (and its pinned
-// tLISI) in an input pool shared by all streams and is copied there through
-// the pool's pinned mirror.  It is then imaged into its ring slot and, from
+// Each stream owns a HIP stream for its launch sequences and one for its
+// input copies (both default priority), work buffers for kServeBatch
+// snapshots, a device ring of `ring` image slots and its window table.  A
+// submitted snapshot reserves room for its inputs (and its pinned tLISI) in
+// an input pool shared by all streams and is copied there through the pool's
+// pinned mirror on the copy stream, which the launch sequence that reads it
+// waits for.  It is then imaged into its ring slot and, from
 // the third snapshot on, triggered against the two previous ring slots (the
 // window of FIPkernels.cu:806-851); the tLISI is copied to pinned memory and
 // an event recorded behind it.  A completion thread polls the events of the
 // snapshots in flight, stamps each completion time and releases the
 // reservation, so no host callback ever stalls a stream.
 //
-// Dynamic batching: a snapshot submitted while its stream's previous launch
-// sequence is still running waits in the stream's queue; when that sequence
-// completes, the completion thread launches the queued snapshots (up to
-// kServeBatch, in consecutive ring slots) as one multi-snapshot sequence.
-// Once a full batch has queued up, it is enqueued at once behind the running
-// sequence (at most two per stream), so a saturated stream keeps its GPU
-// queue filled.  An idle stream launches a snapshot at once, alone, so the
-// latency at low load is that of one snapshot.
+// Dynamic batching: a stream keeps at most two launch sequences enqueued
+// (the running one and the next, so its GPU queue never runs dry while the
+// completion thread relaunches).  A snapshot submitted while two are
+// enqueued waits in the stream's queue; when one completes, the completion
+// thread launches the queued snapshots (up to kServeBatch, in consecutive
+// ring slots) as one multi-snapshot sequence.  An idle stream launches a
+// snapshot at once, alone, so the latency at low load is that of one
+// snapshot.
 //
 // Admission by HBM budget: at creation every stream's fixed device state
 // (work buffers, ring, window table) plus one snapshot of the largest
@@ -142,6 +144,7 @@ namespace {
 struct InSlot {
     size_t off = 0, bytes = 0;   // reservation in the input pool
     hipEvent_t done = nullptr;   // recorded after the tLISI copy
+    hipEvent_t copied = nullptr; // recorded after the input copy (copy stream)
     double arrival = 0.0;
     long index = 0;              // position in its stream
     int ring_slot = 0;           // index % ring: its image slot
@@ -155,7 +158,8 @@ struct SubStream {
     fip_stream_config cfg{};
     Plan* plan = nullptr;
     Tuning tu{};
-    hipStream_t hs = nullptr;
+    hipStream_t hs = nullptr;   // launch sequences
+    hipStream_t cs = nullptr;   // input copies, so they run under the sequences
     DevBuf rec, bkslot, sorted, hist, bstart, kt, pre, work;   // kServeBatch snapshots
     DevBuf batch_in;                                           // gathered inputs (BatchLayout)
     DevBuf img, meta, geom, pmax;                              // ring
@@ -271,6 +275,7 @@ void on_complete(InSlot* sl, double t)
 // flight), then each reservation is released and the first error kept.
 void fail_slots(fip_server* srv, SubStream& ss, const std::vector<InSlot*>& b, int rc)
 {
+    (void)hipStreamSynchronize(ss.cs);
     (void)hipStreamSynchronize(ss.hs);
     {
         std::lock_guard<std::mutex> lk(srv->mu);
@@ -314,6 +319,7 @@ int launch_queued(fip_server* srv, SubStream& ss)
         if (e_ != hipSuccess) return fail(hip_error(e_, #x));   \
     } while (0)
     hipStream_t st = ss.hs;
+    for (InSlot* sl : b) SRV_HIP(hipStreamWaitEvent(st, sl->copied, 0));
     const size_t N = ss.cfg.num_baselines, I = ss.cfg.image_size, plane = I * I;
     const size_t vb = N * esize(ss.cfg.vis_type);
     const size_t un = I / ss.cfg.unit_size;
@@ -365,22 +371,29 @@ int launch_queued(fip_server* srv, SubStream& ss)
                        nullptr, N, (int)N, ss.cfg.vis_type};
     }
     const PlanTables& p = ss.plan->t;
-    // One snapshot that completes a window carries the trigger in its splat
-    // (its image still stored for the next two windows).
-    const bool fuse = k == 1 && b[0]->window >= 0 && splat_trigger_fusable(p, ss.tu, T);
-    if (int rc = imaging_sequence(nullptr, p, w, in, k, (int)N, ss.cfg.cell_size, nullptr, (int)N, ss.tu, st, fuse))
+    int j0 = 0;   // first snapshot of the sequence that completes a window
+    while (j0 < k && b[j0]->window < 0) ++j0;
+    // Every snapshot from j0 on completes a window.  With a 32-pixel trigger
+    // tile each carries its window's trigger in its own splat, launched one
+    // after another (a window's earlier images, from this sequence or an
+    // earlier one, are splatted by then), each image still stored for the
+    // next two windows; the terms buffer is reused window after window.
+    const bool fuse = j0 < k && splat_trigger_fusable(p, ss.tu, T);
+    if (int rc = imaging_sequence(nullptr, p, w, in, k, (int)N, ss.cfg.cell_size, nullptr, (int)N, ss.tu, st,
+                                  fuse ? k - j0 : 0))
         return fail(rc);
     WorkView wr = w;   // the whole ring
     wr.meta = ss.meta.as<SnapMeta>();
     wr.pmax = ss.pmax.as<unsigned>();
     wr.img = ss.img.as<float>();
     float* dout = ss.out.as<float>();
-    int j0 = 0;   // first snapshot of the sequence that completes a window
-    while (j0 < k && b[j0]->window < 0) ++j0;
     if (fuse) {
-        const int sa = (int)((b[0]->index - 2) % ring), sb = (int)((b[0]->index - 1) % ring);
-        SRV_HIP(launch_splat_trigger(w, p, 0, wr.img + (size_t)sa * plane, wr.img + (size_t)sb * plane, true, st));
-        SRV_HIP(launch_trigger_finish(wr, p, sa, sb, r0, T, dout, st));
+        for (int j = j0; j < k; ++j) {
+            const int sa = (int)((b[j]->index - 2) % ring), sb = (int)((b[j]->index - 1) % ring);
+            SRV_HIP(launch_splat_trigger(w, p, j, wr.img + (size_t)sa * plane, wr.img + (size_t)sb * plane, true,
+                                         st));
+            SRV_HIP(launch_trigger_finish(wr, p, sa, sb, r0 + j, T, dout + (size_t)(j - j0) * un * un, st));
+        }
     } else if (j0 < k) {
         // windows (r-2, r-1, r) of ring slots r0+j0 .. r0+k-1, one table row each
         SRV_HIP(launch_trigger(wr, p, ss.win.as<int>() + 3 * (r0 + j0), k - j0, T, dout, st));
@@ -406,21 +419,16 @@ int launch_queued(fip_server* srv, SubStream& ss)
 // Launch sequences per stream enqueued at once: the running one and the next.
 constexpr int kMaxRunning = 2;
 
-// Largest sequence that launch_queued would take from the front of the queue.
-int batch_limit(const fip_server* srv, const SubStream& ss)
-{
-    const int ring = srv->ring, r0 = ss.queue.front()->ring_slot;
-    return std::min({kServeBatch, ring - r0, ring - 2});
-}
 
-// Launches queued snapshots of ss while that is useful: any queue when no
-// sequence is running, a full batch behind a running one.  The caller holds
-// the context lock and srv->qmu.  A failed launch releases the rest of the
-// queue unlaunched and returns its error (also kept in srv->launch_error).
+// Launches the queued snapshots of ss while fewer than kMaxRunning
+// sequences are enqueued, so the stream's next sequence is always waiting
+// behind the running one; what queues up meanwhile goes out together.  The
+// caller holds the context lock and srv->qmu.  A failed launch releases the
+// rest of the queue unlaunched and returns its error (also kept in
+// srv->launch_error).
 int launch_ready(fip_server* srv, SubStream& ss)
 {
-    while (!ss.queue.empty() && ss.running < kMaxRunning &&
-           (ss.running == 0 || (int)ss.queue.size() >= batch_limit(srv, ss))) {
+    while (!ss.queue.empty() && ss.running < kMaxRunning) {
         if (int rc = launch_queued(srv, ss)) {
             fail_slots(srv, ss, ss.queue, rc);
             ss.queue.clear();
@@ -549,6 +557,8 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
             srv->streams.push_back(std::move(ss));
             e = hipStreamCreateWithFlags(&ssp->hs, hipStreamNonBlocking);
             if (e != hipSuccess) return fail(hip_error(e, "hipStreamCreate"));
+            e = hipStreamCreateWithFlags(&ssp->cs, hipStreamNonBlocking);
+            if (e != hipSuccess) return fail(hip_error(e, "hipStreamCreate"));
             const size_t G = ssp->G, M = ssp->M, B = kServeBatch;
             for (auto pr : {std::make_pair(&ssp->rec, B * N * sizeof(VisRec)),
                             std::make_pair(&ssp->sorted, B * N * sizeof(VisRec)),
@@ -600,8 +610,10 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
             if (srv->idle.empty()) {
                 auto fresh = std::make_unique<InSlot>();
                 fresh->srv = srv;
-                const hipError_t e = hipEventCreateWithFlags(&fresh->done, hipEventDisableTiming);
+                hipError_t e = hipEventCreateWithFlags(&fresh->done, hipEventDisableTiming);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&fresh->copied, hipEventDisableTiming);
                 if (e != hipSuccess) {
+                    if (fresh->done) (void)hipEventDestroy(fresh->done);
                     srv->alloc.give(off, ss.lay.bytes);
                     return hip_error(e, "hipEventCreate (server)");
                 }
@@ -622,7 +634,7 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     sl->stream_id = sid;
     bool enqueued = false;   // a copy reading the reservation may be in flight
     auto undo = [&](int rc) {
-        if (enqueued) (void)hipStreamSynchronize(ss.hs);
+        if (enqueued) (void)hipStreamSynchronize(ss.cs);
         std::lock_guard<std::mutex> lk(srv->amu);
         srv->alloc.give(sl->off, sl->bytes);
         srv->idle.push_back(sl);
@@ -654,7 +666,8 @@ int fip_server_submit(fip_server* srv, int sid, const void* vis_re, const void* 
     // ordered before any launch sequence that reads it
     enqueued = true;
     {
-        const hipError_t e = hipMemcpyAsync(d, h, L.res, hipMemcpyHostToDevice, ss.hs);
+        hipError_t e = hipMemcpyAsync(d, h, L.res, hipMemcpyHostToDevice, ss.cs);
+        if (e == hipSuccess) e = hipEventRecord(sl->copied, ss.cs);
         if (e != hipSuccess) return undo(hip_error(e, "hipMemcpyAsync (server input)"));
     }
     ss.submitted += 1;
@@ -752,9 +765,12 @@ void fip_server_destroy(fip_server* srv)
                           &ss->batch_in, &ss->img, &ss->meta, &ss->geom, &ss->pmax, &ss->win, &ss->out})
             b->release();
         if (ss->hs) (void)hipStreamDestroy(ss->hs);
+        if (ss->cs) (void)hipStreamDestroy(ss->cs);
     }
-    for (auto& sl : srv->slots)
+    for (auto& sl : srv->slots) {
         if (sl->done) (void)hipEventDestroy(sl->done);
+        if (sl->copied) (void)hipEventDestroy(sl->copied);
+    }
     srv->pool.release();
     srv->hpool.release();
     delete srv;

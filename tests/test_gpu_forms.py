@@ -304,18 +304,19 @@ def test_serving_streams_match_pipe(fip):
         srv.close()
 
 
-def test_serving_batches_queued_snapshots(fip):
-    """Snapshots submitted back to back queue behind their stream's running
-    launch sequence and go out together, up to three per sequence in
-    consecutive ring slots, never over the two images that the sequence's
-    first window reads.  For every count from 3 to 8 snapshots on a ring of 5
-    (sequences start at every slot and the ring wraps) the latest window must
-    be the same bits as FIpipe of the last three snapshots, at a 32-pixel tile
-    (a one-snapshot sequence carries the trigger in its splat) and at 16
-    (separate trigger), and some sequences must have held several snapshots."""
+def test_serving_batches_queued_snapshots(fip, large):
+    """Snapshots submitted back to back go out in launch sequences of up to
+    three in consecutive ring slots (at most two sequences enqueued per
+    stream), never over the two images that a sequence's first window reads.
+    For every count from 3 to 8 snapshots on a ring of 5 (sequences start at
+    every slot and the ring wraps) the latest window must be the same bits as
+    FIpipe of the last three snapshots, at a 32-pixel tile (a one-snapshot
+    sequence carries the trigger in its splat) and at 16 (separate trigger).
+    At 4096^2 with 130816 baselines a sequence takes longer than a submit, so
+    snapshots queue up: some sequences must hold several, and the latest
+    window must still be the same bits as FIpipe."""
     image, nbl, cell = 1024, 8000, 1.45e-5
     vr, vi, B, V = snapshots(31, 8, nbl, image, cell, transient_snap=4)
-    total = seqs = 0
     for tile in (32, 16):
         cfg = [dict(num_baselines=nbl, image_size=image, cell_size=cell, unit_size=tile)]
         for n in range(3, 9):
@@ -338,9 +339,26 @@ def test_serving_batches_queued_snapshots(fip):
             assert st["processed"] == n and win == n - 3
             assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
             assert 1 <= nseq <= n
-            total += n
-            seqs += nseq
-    assert seqs < total
+    d, w = large, LARGE
+    n = 24   # snapshot k carries the inputs of snapshot k % 3
+    cfg = [dict(num_baselines=d["nbl"], image_size=w["image"], cell_size=w["cell"], unit_size=w["tile"])]
+    srv = fip.Server(cfg, ring=8, hbm_budget_bytes=8 << 30)
+    try:
+        for k in range(n):
+            j = k % 3
+            while not srv.submit(0, d["vis_re"][j], d["vis_im"][j], d["bin"][j], d["vin"][j]):
+                time.sleep(1e-4)
+        srv.drain()
+        st = srv.stats()
+        got, win = srv.latest(0)
+        nseq = srv.sequences()
+    finally:
+        srv.close()
+    ref = fip.fipipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nbl"], w["image"], 3, w["cell"], w["tile"])
+    print(f"4096^2: {n} snapshots in {nseq} sequences, window {win}")
+    assert st["processed"] == n and win == n - 3
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    assert nseq < n
 
 
 def test_serving_admission_per_arrival(fip):
