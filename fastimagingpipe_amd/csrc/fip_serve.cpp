@@ -296,11 +296,11 @@ void fail_slots(fip_server* srv, SubStream& ss, const std::vector<InSlot*>& b, i
 // snapshot, which the completion thread then watches.  The caller holds the
 // context lock and srv->qmu.
 //
-// A sequence images all of its snapshots before its trigger reads the first
-// window, whose two earlier images sit in ring slots r0-2 and r0-1 (mod ring)
-// from earlier sequences; the sequence writes slots r0 .. r0+k-1, so k is at
-// most ring-2 to leave those two untouched, and at most ring-r0 so that its
-// slots are consecutive.
+// The sequence's images go to ring slots r0 .. r0+k-1, and its first window
+// reads the two earlier images in slots r0-2 and r0-1 (mod ring), written by
+// earlier sequences.  k is at most ring-2, so no image of the sequence lands
+// on those two whatever the order of its splats and triggers, and at most
+// ring-r0, so that its slots are consecutive.
 int launch_queued(fip_server* srv, SubStream& ss)
 {
     if (ss.queue.empty()) return FIP_OK;
@@ -418,7 +418,6 @@ int launch_queued(fip_server* srv, SubStream& ss)
 
 // Launch sequences per stream enqueued at once: the running one and the next.
 constexpr int kMaxRunning = 2;
-
 
 // Launches the queued snapshots of ss while fewer than kMaxRunning
 // sequences are enqueued, so the stream's next sequence is always waiting
