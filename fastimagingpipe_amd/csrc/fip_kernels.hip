@@ -690,10 +690,18 @@ __device__ __forceinline__ void stage1_column(int s, int c, int4 rg, const VisRe
     // one per row and column here).
     const int h = I / 2;
     float2* ks = kt + (size_t)s * ((size_t)I * (M + 1));
-#pragma unroll 4
-    for (int yp = t; yp < I; yp += NT) {
+    // Two consecutive rows per thread and one 16-byte store: with yp even the
+    // pair shares a K-tile run (kKTile is even) at a 16-byte offset, and its
+    // LDS slots are adjacent (an even index and the next one never straddle
+    // a padding slot; h and G are even, so y and its wrap keep the parity).
+    static_assert(kKTile % 2 == 0, "row pairs inside a K-tile run");
+#pragma unroll 2
+    for (int yp = 2 * t; yp < I; yp += 2 * NT) {
         const int y = yp - h;
-        ks[((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)] = fb[lds_pad(y < 0 ? y + G : y)];
+        const int q = lds_pad(y < 0 ? y + G : y);
+        const float2 v0 = fb[q], v1 = fb[q + 1];
+        *reinterpret_cast<float4*>(ks + ((size_t)(yp / kKTile) * (M + 1) + c) * kKTile + (yp % kKTile)) =
+            make_float4(v0.x, v0.y, v1.x, v1.y);
     }
 }
 
