@@ -574,6 +574,13 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
                 e = pr.first->ensure(pr.second);
                 if (e != hipSuccess) return fail(hip_error(e, "hipMalloc (server)"));
             }
+            // Every buffer is written once here, so the first snapshots after
+            // creation do not pay the first use of freshly allocated memory.
+            for (DevBuf* b : {&ssp->rec, &ssp->bkslot, &ssp->sorted, &ssp->hist, &ssp->bstart, &ssp->kt, &ssp->pre,
+                              &ssp->work, &ssp->batch_in, &ssp->img, &ssp->meta, &ssp->geom, &ssp->pmax, &ssp->out}) {
+                e = hipMemsetAsync(b->p, 0, b->cap, ssp->hs);
+                if (e != hipSuccess) return fail(hip_error(e, "hipMemsetAsync (server)"));
+            }
             // Window table: window w (snapshot index >= 2) uses ring slots
             // (w-2, w-1, w) mod ring; store all ring rotations.
             std::vector<int> wt(3 * ring);
@@ -585,6 +592,14 @@ int fip_server_create(const fip_stream_config* streams, int nstreams, int ring, 
             e = hipMemcpy(ssp->win.p, wt.data(), wt.size() * sizeof(int), hipMemcpyHostToDevice);
             if (e != hipSuccess) return fail(hip_error(e, "hipMemcpy (server)"));
             ssp->latest.assign(un * un, 0.0f);
+        }
+        // The input pool and its pinned mirror are first written here too.
+        std::memset(srv->hpool.p, 0, srv->pool_bytes);
+        e = hipMemsetAsync(srv->pool.p, 0, srv->pool_bytes, srv->streams[0]->hs);
+        if (e != hipSuccess) return fail(hip_error(e, "hipMemsetAsync (server pool)"));
+        for (auto& ss : srv->streams) {
+            e = hipStreamSynchronize(ss->hs);
+            if (e != hipSuccess) return fail(hip_error(e, "hipStreamSynchronize (server)"));
         }
         *out = srv;
         return FIP_OK;
