@@ -365,6 +365,8 @@ def main() -> int:
                 batched[f"{image}_{name}_snapshots_per_s"] = round(rate, 1)
                 if image == 4096:
                     batched[f"{image}_{name}_vs_plain"] = round(rate / value, 3)
+                    # trigger windows per second: 62 per batch of 64, 1 per plain 3-snapshot step
+                    batched[f"{image}_{name}_windows_vs_plain"] = round((rate * 62 / 64) / (value / 3), 3)
                 log(f"batched {image}^2 {name}: {rate:.1f} snapshots/s")
         extra["batched"] = batched
         log("serving form ...")
