@@ -67,7 +67,8 @@ struct SnapGeom {
     double curv;                 // bound on the SIN map's curvature (1/px), <0: unbounded
     double kappa;                // bound on |grad| of the SIN shift over the image
     int sin_on;                  // xi^2 + eta^2 != 0
-    int series;                  // every direction cosine has r^2 <= 0.01: n' by its power series
+    int series;                  // every direction cosine has r^2 <= 0.01 and the splat's quadratic
+                                 // model of the series along a group is within 5e-8 px: n' by its series
     int ccap;                    // upper bound on the sources whose splat reaches one pixel
     int pad;
 };

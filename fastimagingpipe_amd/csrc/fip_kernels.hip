@@ -230,7 +230,18 @@ __global__ void k_reset(const float* __restrict__ vin, const int* __restrict__ n
         }
         // n' = 1 - sqrt(1 - r^2) = r^2 (1/2 + r^2/8 + r^4/16 + 5 r^6/128 + ...);
         // for r^2 <= 0.01 the terms after r^8 are below 3e-12 of n'.
-        gm.series = (rmax * rmax <= 0.01) ? 1 : 0;
+        // The splat takes this series along each group of 8 consecutive
+        // sources of a row (k = 0..7) from its quadratic Taylor model in k.
+        // r^2 is quadratic in k with |dr^2/dk| <= 2 r d and d2r^2/dk2 = 2 d^2,
+        // d = cell |(m00, m10)|; for r <= 0.1 the series' third k-derivative
+        // is at most 0.31 d^3, so the model's remainder at k <= 7 is at most
+        // 17.6 d^3 in n', 17.6 d^3 max(|sx|, |sy|) in pixels.  The series
+        // form is taken only where that is at most 5e-8 px (a tenth of an FP32
+        // ulp of the splat's local positions); otherwise the closed form per
+        // source.
+        const double dcol = delta * sqrt(gm.m00 * gm.m00 + gm.m10 * gm.m10);
+        const double qerr = 18.0 * dcol * dcol * dcol * fmax(fabs(gm.sx), fabs(gm.sy));
+        gm.series = (rmax * rmax <= 0.01 && qerr <= 5e-8) ? 1 : 0;
         gm.pad = 0;
         geom[i] = gm;
     }
@@ -1308,7 +1319,6 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
     const float iscale = ldexpf(1.0f, -ks);
     const GeomF gf = geom_f(g);
     const float* ps = pre + (size_t)s * I * I;
-    const double ql1 = qb[0], qh1 = qb[1], ql2 = qb[2], qh2 = qb[3];
     for (int rb = 0; rb < H; rb += ROW_CAP) {
         const int nrow = min(ROW_CAP, H - rb);
         // Per source row: the i range whose affine position lies in the q box.
@@ -1316,7 +1326,9 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
             const double b = (double)(j0 + rb + r - h);
             double alo = -1e300, ahi = 1e300;
             const double mA[2] = {g.m00, g.m10}, mB[2] = {g.m01, g.m11}, iA[2] = {qb[4], qb[5]};
-            const double lo[2] = {ql1, ql2}, hi[2] = {qh1, qh2};
+            // The q box is read from LDS here rather than held in registers
+            // across the source loop (eight VGPRs; holding them spilled).
+            const double lo[2] = {qb[0], qb[2]}, hi[2] = {qb[1], qb[3]};
             for (int k = 0; k < 2; ++k) {
                 const double cst = mB[k] * b;
                 if (fabs(mA[k]) > 1e-12) {
@@ -1432,22 +1444,52 @@ __global__ __launch_bounds__(256, 8) void k_splat(const float* __restrict__ pre,
                     // rounded once from FP64.
                     const float l0 = (float)fma(-g.delta, q1d, g.delta), dl = -gf.delta * gf.m00;
                     const float n0 = (float)fma(g.delta, q2d, -g.delta), dn = gf.delta * gf.m10;
+                    // Series form: source k of the group lands at
+                    // (Q1 + B1 k + C1 k^2, Q2 + B2 k + C2 k^2) relative to the
+                    // origin, the affine step plus the quadratic Taylor model in
+                    // k of the SIN shift (sx, sy) n'(k) (its remainder is
+                    // bounded in k_reset).  With r^2(k) = R0 + R1 k + R2 k^2 and
+                    // the series f(x) = x/2 + x^2/8 + x^3/16 + 5x^4/128,
+                    // n'(k) ~ f(R0) + f'(R0) R1 k + (f'(R0) R2 + f''(R0) R1^2 / 2) k^2.
+                    float Q1 = r1, B1 = gf.m00, C1 = 0.0f, Q2 = r2, B2 = gf.m10, C2 = 0.0f;
+                    if constexpr (ser == 1) {
+                        const float R0 = fmaf(l0, l0, n0 * n0);
+                        const float R1 = fmaf(l0, 2.0f * dl, n0 * (2.0f * dn));
+                        const float R2 = fmaf(dl, dl, dn * dn);
+                        const float f0 = R0 * fmaf(R0, fmaf(R0, fmaf(R0, 0.0390625f, 0.0625f), 0.125f), 0.5f);
+                        const float f1 = fmaf(R0, fmaf(R0, fmaf(R0, 0.15625f, 0.1875f), 0.25f), 0.5f);
+                        const float f2h = fmaf(R0, fmaf(R0, 0.234375f, 0.1875f), 0.125f);
+                        const float b = f1 * R1, c = fmaf(f1, R2, f2h * R1 * R1);
+                        Q1 = fmaf(gf.sx, f0, r1);
+                        Q2 = fmaf(gf.sy, f0, r2);
+                        B1 = fmaf(gf.sx, b, gf.m00);
+                        B2 = fmaf(gf.sy, b, gf.m10);
+                        C1 = gf.sx * c;
+                        C2 = gf.sy * c;
+                    }
 #pragma unroll
                     for (int k = 0; k < GN; ++k) {
-                        float o1 = fmaf((float)k, gf.m00, r1), o2 = fmaf((float)k, gf.m10, r2);
-                        if (ser >= 0 || gf.sin_on) {
-                            const float l = fmaf((float)k, dl, l0), mm = fmaf((float)k, dn, n0);
-                            const float rr = l * l + mm * mm;
-                            float np = 0.0f;
-                            if (ser == 1 || (ser < 0 && gf.series)) {
-                                // every source of the image has r^2 <= 0.01 (checked in k_reset)
-                                np = rr * fmaf(rr, fmaf(rr, fmaf(rr, 0.0390625f, 0.0625f), 0.125f), 0.5f);
-                            } else if (rr <= 1.0f) {
-                                // 1 - sqrt(1 - r^2) = r^2 / (1 + sqrt(1 - r^2)); hardware sqrt/rcp
-                                np = rr * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - rr));
+                        float o1, o2;
+                        if constexpr (ser == 1) {
+                            o1 = k == 0 ? Q1 : fmaf((float)(k * k), C1, fmaf((float)k, B1, Q1));
+                            o2 = k == 0 ? Q2 : fmaf((float)(k * k), C2, fmaf((float)k, B2, Q2));
+                        } else {
+                            o1 = fmaf((float)k, gf.m00, r1);
+                            o2 = fmaf((float)k, gf.m10, r2);
+                            if (ser == 0 || gf.sin_on) {
+                                const float l = fmaf((float)k, dl, l0), mm = fmaf((float)k, dn, n0);
+                                const float rr = l * l + mm * mm;
+                                float np = 0.0f;
+                                if (ser < 0 && gf.series) {
+                                    // every source of the image has r^2 <= 0.01 (checked in k_reset)
+                                    np = rr * fmaf(rr, fmaf(rr, fmaf(rr, 0.0390625f, 0.0625f), 0.125f), 0.5f);
+                                } else if (rr <= 1.0f) {
+                                    // 1 - sqrt(1 - r^2) = r^2 / (1 + sqrt(1 - r^2)); hardware sqrt/rcp
+                                    np = rr * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - rr));
+                                }
+                                o1 = fmaf(gf.sx, np, o1);
+                                o2 = fmaf(gf.sy, np, o2);
                             }
-                            o1 = fmaf(gf.sx, np, o1);
-                            o2 = fmaf(gf.sy, np, o2);
                         }
                         bool fin = true;
                         if constexpr (unb) {

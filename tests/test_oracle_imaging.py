@@ -409,3 +409,31 @@ def test_splat_first_moments_are_linear(oracle):
     assert out.sum() == pytest.approx(w.sum(), rel=1e-13)
     assert (out * cols).sum() == pytest.approx((w * idx[..., 0]).sum(), rel=1e-13)
     assert (out * rows).sum() == pytest.approx((w * idx[..., 1]).sum(), rel=1e-13)
+
+
+@pytest.mark.parametrize("image,nbl,cell", [(128, 900, 1e-4), (256, 2016, 5.8e-5)])
+def test_gpu_image_cases_resolve_the_splat_group_curvature(oracle, image, nbl, cell):
+    """The splat places the 8 sources of a row group (columns i0..i0+7) from a
+    quadratic model in k = i - i0 of the SIN displacement D(i) = SIN position -
+    affine position.  Its k^2 term is at most ~5e-4 px at these sizes.  Shifting
+    every source by minus that term (D''/2 k^2, from second differences of the
+    oracle's own map) moves the oracle image of each snapshot of
+    test_image_matches_oracle by more than twice that test's 5e-5 rel-rms
+    tolerance (1.4e-4 to 4.1e-4), so the GPU test would catch a dropped or wrong
+    k^2 term.  Those snapshots take the series form at 128^2 and 256^2 (model
+    remainder bound 1.2e-8 to 4.4e-8 px, within the 5e-8 px limit) and the
+    closed form per source at 64^2 (bound ~1.9e-7 px)."""
+    from tests.inputs import rel_rms, snapshots
+
+    vr, vi, B, V = snapshots(11 + image, 3, nbl, image, cell)
+    k = (np.arange(image) % 8)[:, None, None]
+    for s in range(3):
+        pre, _, _ = oracle.image(vr[s], vi[s], B[s], V[s], image, cell, want_pre=True)
+        sin = oracle.coord_map(V[s], image, cell, sin_projection=True)
+        disp = sin - oracle.coord_map(V[s], image, cell, sin_projection=False)
+        o = disp[0::8]                                   # [group, j, 2] at i0, i0+1, i0+2
+        half_d2 = 0.5 * (disp[2::8] - 2.0 * disp[1::8] + o)
+        shifted = sin - np.repeat(half_d2, 8, axis=0) * k ** 2
+        base = oracle.splat(sin, pre, nbl)
+        moved = rel_rms(oracle.splat(shifted, pre, nbl), base)
+        assert moved > 1e-4, (s, moved)
