@@ -19,7 +19,8 @@ def test_lds_fft_matches_numpy(fip, n):
     assert err < 2e-6 * np.log2(n), err
 
 
-@pytest.mark.parametrize("image,nbl,cell", [(64, 300, 2e-4), (128, 900, 1e-4), (256, 2016, 5.8e-5)])
+@pytest.mark.parametrize("image,nbl,cell", [(64, 300, 2e-4), (128, 900, 1e-4), (256, 2016, 5.8e-5),
+                                             (512, 4000, 2.9e-5), (1024, 8000, 1.45e-5)])
 def test_image_matches_oracle(fip, oracle, image, nbl, cell):
     vr, vi, B, V = snapshots(11 + image, 3, nbl, image, cell)
     pre, img, mx = fip.image(vr, vi, B, V, nbl, image, 3, cell)
@@ -28,9 +29,11 @@ def test_image_matches_oracle(fip, oracle, image, nbl, cell):
         e_pre = rel_rms(pre[s], opre)
         e_img = rel_rms(img[s], oimg)
         print(f"I={image} s={s} pre rel-rms {e_pre:.2e} img rel-rms {e_img:.2e} max {mx[s]:.6g}/{omax:.6g}")
-        assert e_pre < 2e-5
-        assert e_img < 5e-5
-        assert abs(mx[s] - omax) <= 1e-4 * omax
+        # Measured 6.7e-7 to 8.6e-7 (pre) and 5.5e-7 to 2.1e-6 (img) over these
+        # cases; the bars keep about a five-fold margin.
+        assert e_pre < 5e-6
+        assert e_img < 1e-5
+        assert abs(mx[s] - omax) <= 1e-5 * omax
 
 
 def test_pipe_matches_oracle_small(fip, oracle):

@@ -418,7 +418,7 @@ def test_gpu_image_cases_resolve_the_splat_group_curvature(oracle, image, nbl, c
     affine position.  Its k^2 term is at most ~5e-4 px at these sizes.  Shifting
     every source by minus that term (D''/2 k^2, from second differences of the
     oracle's own map) moves the oracle image of each snapshot of
-    test_image_matches_oracle by more than twice that test's 5e-5 rel-rms
+    test_image_matches_oracle by more than ten times that test's 1e-5 rel-rms
     tolerance (1.4e-4 to 4.1e-4), so the GPU test would catch a dropped or wrong
     k^2 term.  Those snapshots take the series form at 128^2 and 256^2 (model
     remainder bound 1.2e-8 to 4.4e-8 px, within the 5e-8 px limit) and the
