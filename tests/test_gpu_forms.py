@@ -38,7 +38,7 @@ def test_small_array_pipe_matches_oracle(fip, oracle):
     got, ref = _pipe_both(fip, oracle, d, SMALL)
     err = float(np.abs(got - ref).max())
     print("tLISI max abs err", err, "min", ref.min())
-    assert err < 2e-5
+    assert err < 1e-6
     _check_trigger_sets(got, ref, err)
 
 
@@ -53,7 +53,7 @@ def test_large_array_pipe_matches_oracle(fip, oracle, large):
     got, ref = _pipe_both(fip, oracle, large, LARGE)
     err = float(np.abs(got - ref).max())
     print(f"tLISI max abs err {err:.3g}, min tLISI {ref.min():.6f} (oracle {time.time() - t0:.1f} s)")
-    assert err < 2e-5
+    assert err < 1e-6
     _check_trigger_sets(got, ref, err)
 
 
@@ -92,7 +92,7 @@ def test_other_image_sizes_pipe(fip, oracle, nant, radius, image, cell):
     err = float(np.abs(got - ref).max())
     print(f"{image}^2 {d['nbl']} bl: tLISI max abs err {err:.3g}, min {ref.min():.6f} "
           f"({time.time() - t0:.1f} s)")
-    assert err < 2e-5
+    assert err < 1e-6
     _check_trigger_sets(got, ref, err)
     s = 2
     _, img, mx = fip.image(d["vis_re"][s:s + 1], d["vis_im"][s:s + 1], d["bin"][s:s + 1], d["vin"][s:s + 1],
@@ -139,7 +139,7 @@ def test_precision_form(fip, oracle, vt, which, large):
     e_q = float(np.abs(got - ref_q).max())
     e_full = float(np.abs(got - ref).max())
     print(f"{which} {vt}: tLISI err vs oracle(stored) {e_q:.2e}, vs oracle(fp32 input) {e_full:.2e}")
-    assert e_q < 2e-5
+    assert e_q < 1e-6
     _check_trigger_sets(got, ref, e_full)
     s = 2
     _, img, _ = fip.image(qr[s:s + 1], qi[s:s + 1], d["bin"][s:s + 1], d["vin"][s:s + 1], nbl,
@@ -178,7 +178,9 @@ def test_batch_host_matches_per_window_pipe(fip, oracle):
         assert np.abs(out[k] - one).max() < 1e-6, k
     ref = oracle.pipe(d["vis_re"][2:5], d["vis_im"][2:5], d["bin"][2:5], d["vin"][2:5], nbl, w["image"],
                       3, w["cell"], w["tile"])
-    assert np.abs(out[2] - ref).max() < 2e-5
+    err = float(np.abs(out[2] - ref).max())
+    print(f"batch window 2 vs FP64 oracle: tLISI max abs err {err:.2e}")
+    assert err < 1e-6
 
 
 def test_batch_device_graph_replays(fip):
@@ -220,7 +222,9 @@ def test_batch_uneven_counts_match_compacted_oracle(fip, oracle):
     out = fip.batch_host(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nv, nbl, w["image"], w["cell"],
                          w["tile"])
     ref = oracle.pipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
-    assert np.abs(out[0] - ref).max() < 2e-5
+    err = float(np.abs(out[0] - ref).max())
+    print(f"uneven-count batch vs compacted FP64 oracle: tLISI max abs err {err:.2e}")
+    assert err < 1e-6
 
 
 def test_projection_matches_fixtures_and_oracle(fip, oracle):
@@ -531,7 +535,11 @@ def test_tile_sizes_including_ragged(fip, oracle, tile):
     got = fip.fipipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nbl"], w["image"], 3, w["cell"], tile)
     ref = oracle.pipe(d["vis_re"], d["vis_im"], d["bin"], d["vin"], d["nbl"], w["image"], 3, w["cell"], tile)
     assert got.shape == (w["image"] // tile,) * 2
-    assert np.abs(got - ref).max() < 2e-5
+    err = float(np.abs(got - ref).max())
+    print(f"tile {tile}: tLISI max abs err {err:.2e}")
+    # A one-pixel tile's statistic is a ratio of single-pixel differences,
+    # without the averaging of larger tiles (measured 1.1e-6 at tile 1).
+    assert err < (1e-5 if tile == 1 else 1e-6)
 
 
 def test_extra_snapshots_are_ignored(fip):
@@ -556,7 +564,9 @@ def test_out_of_grid_baselines_only_count_in_the_normalisation(fip, oracle):
     B[:, 40:60, 1] = -1e9
     got = fip.fipipe(d["vis_re"], d["vis_im"], B, d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
     ref = oracle.pipe(d["vis_re"], d["vis_im"], B, d["vin"], nbl, w["image"], 3, w["cell"], w["tile"])
-    assert np.abs(got - ref).max() < 2e-5
+    err = float(np.abs(got - ref).max())
+    print(f"out-of-grid baselines: tLISI max abs err {err:.2e}")
+    assert err < 1e-6
 
 
 # ---------------------------------------------------------------- full workloads
@@ -598,7 +608,7 @@ def test_batch_64_flagged_snapshots_full_workload(fip, oracle, image, cell):
         ref = oracle.pipe(d["vis_re"][sl], d["vis_im"][sl], d["bin"][sl], d["vin"][sl], nbl, image, 3, cell, 32)
         err = float(np.abs(got[k] - ref).max())
         print(f"  window {k}: tLISI max abs err vs FP64 oracle {err:.2e}, min {ref.min():.5f}")
-        assert err < 2e-5
+        assert err < 1e-6
         _check_trigger_sets(got[k], ref, err)
 
 
@@ -655,7 +665,7 @@ def test_serving_subarrays_full_workload(fip, oracle):
             ref = oracle.pipe(qr, qi, d["bin"][js], d["vin"][js], d["nbl"], image, 3, cell, 32)
             err = float(np.abs(tl - ref).max())
             print(f"  stream {k} ({image}^2, {d['nbl']} bl): window {win}, tLISI err vs FP64 oracle {err:.2e}")
-            assert err < 2e-5
+            assert err < 1e-6
             _check_trigger_sets(tl, ref, err)
     finally:
         srv.close()
