@@ -68,7 +68,8 @@ def test_large_array_image_rms_and_peak(fip, oracle, large):
     e_pre, e_img = rel_rms(pre[0], opre), rel_rms(img[0], oimg)
     print(f"4096^2 snapshot 2: pre rel-rms {e_pre:.2e}, img rel-rms {e_img:.2e}, "
           f"max {mx[0]:.7g} / {omax:.7g}")
-    assert e_pre < 1e-5 and e_img < 1e-4
+    # measured 2.1e-6 and 2.3e-6; the contract's image bar is 1e-4
+    assert e_pre < 1e-5 and e_img < 2e-5
     assert np.unravel_index(np.argmax(img[0]), img[0].shape) == np.unravel_index(np.argmax(oimg), oimg.shape)
     assert abs(mx[0] - omax) <= 1e-5 * omax
 
@@ -100,7 +101,8 @@ def test_other_image_sizes_pipe(fip, oracle, nant, radius, image, cell):
     oimg, omax = oracle.image(d["vis_re"][s], d["vis_im"][s], d["bin"][s], d["vin"][s], image, cell)
     e = rel_rms(img[0], oimg)
     print(f"  image rel-rms {e:.2e}, max {mx[0]:.7g} / {omax:.7g}")
-    assert e < 1e-4 and abs(mx[0] - omax) <= 1e-5 * omax
+    # measured 1.3e-6 (2048^2) and 5.5e-6 (8192^2); the contract's image bar is 1e-4
+    assert e < 5e-5 and abs(mx[0] - omax) <= 1e-5 * omax
 
 
 # Bound on the image error that storing visibilities in 2 bytes allows:
